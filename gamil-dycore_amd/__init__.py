@@ -1,0 +1,283 @@
+"""gamil-dycore_amd -- MI355X-native barotropic time step of gamil-dycore.
+
+Python mirror of the C ABI in include/gamil_dycore.h (ctypes over the in-tree
+libgamil_dycore.so).  The object model follows the reference's dycore_mod
+facade (src/dycore_mod.F90:22-25): construct with the &dycore_params values
+(dycore_init), hand over the initial state (the IC modules' writes into
+state(1)), then step (the dycore_run loop body) and read the state back.
+
+There is no CPU fallback: every compute call runs the HIP kernels, and
+importing the module fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgamil_dycore.so")
+
+GD_OK, GD_ERR_INVALID_ARG, GD_ERR_HIP, GD_ERR_NAN, GD_ERR_NOT_SUPPORTED = range(5)
+SPLIT = {"none": 0, "csp1": 1, "csp2": 2, "isp": 3}
+ADV = {"center_diff": 0, "upwind": 1, "weno": 2}
+PASS = {"all": 0, "fast": 1, "slow": 2}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_bf16": 2}
+
+
+class DycoreError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[Error]: {msg} (code {code})")
+        self.code = code
+
+
+class _Config(C.Structure):
+    _fields_ = [
+        ("num_lon", C.c_int32), ("num_lat", C.c_int32), ("time_step_size", C.c_double),
+        ("split_scheme", C.c_int32), ("subcycles", C.c_int32), ("qcon_modified", C.c_int32),
+        ("uv_adv_scheme", C.c_int32), ("uv_adv_upwind_lon_beta", C.c_double),
+        ("uv_adv_upwind_lat_beta", C.c_double), ("use_diffusion", C.c_int32),
+        ("diffusion_order", C.c_int32), ("diffusion_coef", C.c_double),
+        ("use_zonal_tend_filter", C.c_int32), ("n_cutoff", C.c_int32),
+        ("cutoff", C.POINTER(C.c_int32)), ("precision", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+class ICParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("nmax", C.c_int32), ("slope", C.c_double),
+        ("rms_height", C.c_double), ("mean_depth", C.c_double), ("n_mountains", C.c_int32),
+        ("mountain_seed", C.c_uint64), ("pert_seed", C.c_uint64), ("pert_nmax", C.c_int32),
+        ("pert_rms", C.c_double),
+    ]
+
+
+# every symbol declared in include/gamil_dycore.h
+EXPORTS = (
+    "gd_create", "gd_destroy", "gd_last_error", "gd_set_state", "gd_diag", "gd_step",
+    "gd_get_state", "gd_get_diag_fields", "gd_space_operators", "gd_predict_correct",
+    "gd_ordinary_diffusion", "gd_filter_rows", "gd_filter_table", "gd_ic_balanced",
+    "gd_ic_rossby_haurwitz", "gd_ic_mountain_zonal_flow", "gd_profile_kernels",
+    "gd_device_bytes", "gd_selftest_mfma",
+)
+
+_lib = None
+
+
+def lib():
+    """Load libgamil_dycore.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    vp = C.c_void_p
+    i64 = C.c_int64
+    L.gd_create.argtypes = [C.POINTER(_Config), C.c_int32, C.POINTER(vp)]
+    L.gd_destroy.argtypes = [vp]
+    L.gd_last_error.restype = C.c_char_p
+    L.gd_last_error.argtypes = [vp]
+    L.gd_set_state.argtypes = [vp, C.c_int32, dp, dp, dp, dp, i64]
+    L.gd_diag.argtypes = [vp, dp, dp]
+    L.gd_step.argtypes = [vp, C.c_int32, dp, dp, dp, ip]
+    L.gd_get_state.argtypes = [vp, C.c_int32, dp, dp, dp, i64]
+    L.gd_get_diag_fields.argtypes = [vp, C.c_int32, dp, dp, i64]
+    L.gd_space_operators.argtypes = [vp, C.c_int32, C.c_int32, dp, dp, dp, i64]
+    L.gd_predict_correct.argtypes = [vp, C.c_double, C.c_int32, C.c_int32, dp]
+    L.gd_ordinary_diffusion.argtypes = [vp, C.c_double]
+    L.gd_filter_rows.argtypes = [vp, C.c_int32, dp, i64]
+    L.gd_filter_table.argtypes = [vp, ip, ip, ip, ip]
+    L.gd_ic_balanced.argtypes = [C.c_int32, C.c_int32, C.POINTER(ICParams), dp, dp, dp, dp, i64]
+    L.gd_ic_rossby_haurwitz.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                        C.c_double, dp, dp, dp, dp, i64]
+    L.gd_ic_mountain_zonal_flow.argtypes = [C.c_int32, C.c_int32, dp, dp, dp, dp, i64]
+    L.gd_profile_kernels.argtypes = [vp, C.c_int32, C.c_int32, ip, C.POINTER(C.c_int64), dp,
+                                     C.c_char_p, C.c_int32]
+    L.gd_device_bytes.restype = C.c_int64
+    L.gd_device_bytes.argtypes = [vp]
+    L.gd_selftest_mfma.argtypes = [dp, dp, dp, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float)]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _check(code, h=None):
+    if code != GD_OK:
+        msg = lib().gd_last_error(h).decode(errors="replace")
+        raise DycoreError(code, msg)
+
+
+# ---------------------------------------------------------------------------
+# initial conditions (host set-up, not on the time step)
+# ---------------------------------------------------------------------------
+
+def ic_balanced(num_lon, num_lat, seed=1, nmax=20, slope=-3.0, rms_height=150.0,
+                mean_depth=8000.0, n_mountains=0, mountain_seed=0, pert_seed=0, pert_nmax=0,
+                pert_rms=0.0):
+    """Seeded balanced spherical-harmonic state: (u, v, gd, ghs) dense arrays."""
+    N, M = int(num_lon), int(num_lat)
+    u = np.zeros((M, N)); v = np.zeros((M - 1, N)); gd = np.zeros((M, N)); ghs = np.zeros((M, N))
+    p = ICParams(seed, nmax, slope, rms_height, mean_depth, n_mountains, mountain_seed,
+                 pert_seed, pert_nmax, pert_rms)
+    _check(lib().gd_ic_balanced(N, M, C.byref(p), _dp(u), _dp(v), _dp(gd), _dp(ghs), N))
+    return u, v, gd, ghs
+
+
+def ic_rossby_haurwitz(num_lon, num_lat, R=4.0, omg=7.848e-6, gd0=8.0e3 * 9.80616):
+    N, M = int(num_lon), int(num_lat)
+    u = np.zeros((M, N)); v = np.zeros((M - 1, N)); gd = np.zeros((M, N)); ghs = np.zeros((M, N))
+    _check(lib().gd_ic_rossby_haurwitz(N, M, R, omg, gd0, _dp(u), _dp(v), _dp(gd), _dp(ghs), N))
+    return u, v, gd, ghs
+
+
+def ic_mountain_zonal_flow(num_lon, num_lat):
+    N, M = int(num_lon), int(num_lat)
+    u = np.zeros((M, N)); v = np.zeros((M - 1, N)); gd = np.zeros((M, N)); ghs = np.zeros((M, N))
+    _check(lib().gd_ic_mountain_zonal_flow(N, M, _dp(u), _dp(v), _dp(gd), _dp(ghs), N))
+    return u, v, gd, ghs
+
+
+def cutoff_from_lat0(num_lon, num_lat, lat0_deg):
+    """Polar filter table whose kept wavenumber falls toward the poles as
+    cos(lat)/cos(lat0): entry k (k = 1.. from each pole) drives full row 1+k,
+    c_k = floor(num_lon/2 * cos(lat_{1+k}) / cos(lat0)) for |lat| > lat0."""
+    N, M = int(num_lon), int(num_lat)
+    dlat = np.pi / (M - 1)
+    c0 = np.cos(np.deg2rad(lat0_deg))
+    out = []
+    for k in range(1, M):
+        lat = -0.5 * np.pi + k * dlat  # full row 1+k (1-based) from the south pole
+        if abs(np.rad2deg(lat)) <= lat0_deg:
+            break
+        c = int(np.floor(N / 2 * np.cos(lat) / c0))
+        c = max(1, min(c, N // 2 - 1))
+        out.append(c)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the dycore
+# ---------------------------------------------------------------------------
+
+class Dycore:
+    """A device-resident ensemble of `members` runs of one &dycore_params config."""
+
+    def __init__(self, num_lon, num_lat, time_step_size, split_scheme="csp2", subcycles=4,
+                 qcon_modified=True, uv_adv_scheme="center_diff", uv_adv_upwind_lon_beta=0.0,
+                 uv_adv_upwind_lat_beta=0.5, use_diffusion=False, diffusion_order=2,
+                 diffusion_coef=0.0, use_zonal_tend_filter=True,
+                 zonal_tend_filter_cutoff_wavenumber=(), precision="fp64", members=1, device=0):
+        L = lib()
+        self.N, self.M, self.members = int(num_lon), int(num_lat), int(members)
+        self.dt = float(time_step_size)
+        cut = list(zonal_tend_filter_cutoff_wavenumber)
+        self._cut = np.ascontiguousarray(np.asarray(cut or [0], dtype=np.int32))
+        cfg = _Config(self.N, self.M, self.dt, SPLIT[split_scheme], int(subcycles),
+                      int(bool(qcon_modified)), ADV[uv_adv_scheme], float(uv_adv_upwind_lon_beta),
+                      float(uv_adv_upwind_lat_beta), int(bool(use_diffusion)), int(diffusion_order),
+                      float(diffusion_coef), int(bool(use_zonal_tend_filter)), len(cut),
+                      self._cut.ctypes.data_as(C.POINTER(C.c_int32)), PRECISION[precision],
+                      int(device))
+        h = C.c_void_p()
+        _check(L.gd_create(C.byref(cfg), self.members, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state hand-over ------------------------------------------------
+    def set_state(self, u, v, gd, ghs, member=0):
+        f = lambda a, r: np.ascontiguousarray(a, dtype=np.float64).reshape(r, self.N)
+        u, v, gd, ghs = f(u, self.M), f(v, self.M - 1), f(gd, self.M), f(ghs, self.M)
+        _check(lib().gd_set_state(self.h, member, _dp(u), _dp(v), _dp(gd), _dp(ghs), self.N), self.h)
+
+    def get_state(self, member=0):
+        u = np.zeros((self.M, self.N)); v = np.zeros((self.M - 1, self.N)); gd = np.zeros((self.M, self.N))
+        _check(lib().gd_get_state(self.h, member, _dp(u), _dp(v), _dp(gd), self.N), self.h)
+        return u, v, gd
+
+    def diag_fields(self, member=0):
+        div = np.zeros((self.M, self.N)); vor = np.zeros((self.M - 1, self.N))
+        _check(lib().gd_get_diag_fields(self.h, member, _dp(div), _dp(vor), self.N), self.h)
+        return div, vor
+
+    def diag(self):
+        tm = np.zeros(self.members); te = np.zeros(self.members)
+        _check(lib().gd_diag(self.h, _dp(tm), _dp(te)), self.h)
+        return tm, te
+
+    # -- time stepping ----------------------------------------------------
+    def step(self, nsteps, raise_on_nan=True):
+        n = int(nsteps)
+        mass = np.zeros((n, self.members)); energy = np.zeros((n, self.members))
+        beta = np.zeros((n, self.members)); ns = C.c_int32(0)
+        code = lib().gd_step(self.h, n, _dp(mass), _dp(energy), _dp(beta), C.byref(ns))
+        if code == GD_ERR_NAN and not raise_on_nan:
+            return mass, energy, beta, ns.value
+        _check(code, self.h)
+        return mass, energy, beta, ns.value
+
+    # -- operator seams ----------------------------------------------------
+    def space_operators(self, pass_="all", member=0):
+        du = np.zeros((self.M, self.N)); dv = np.zeros((self.M - 1, self.N)); dgd = np.zeros((self.M, self.N))
+        _check(lib().gd_space_operators(self.h, member, PASS[pass_], _dp(du), _dp(dv), _dp(dgd), self.N), self.h)
+        return du, dv, dgd
+
+    def predict_correct(self, dt, pass_="all", qcon_modified=True):
+        beta = np.zeros(self.members)
+        _check(lib().gd_predict_correct(self.h, float(dt), PASS[pass_], int(bool(qcon_modified)), _dp(beta)), self.h)
+        return beta
+
+    def ordinary_diffusion(self, dt):
+        _check(lib().gd_ordinary_diffusion(self.h, float(dt)), self.h)
+
+    def filter_rows(self, x, half=False):
+        y = np.ascontiguousarray(x, dtype=np.float64).copy()
+        _check(lib().gd_filter_rows(self.h, int(bool(half)), _dp(y), self.N), self.h)
+        return y
+
+    def filter_table(self):
+        ff = np.zeros(self.M, np.int32); hf = np.zeros(self.M - 1, np.int32)
+        fc = np.zeros(self.M, np.int32); hc = np.zeros(self.M - 1, np.int32)
+        ipp = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        _check(lib().gd_filter_table(self.h, ipp(ff), ipp(hf), ipp(fc), ipp(hc)), self.h)
+        return ff, hf, fc, hc
+
+    def profile_kernels(self, nsteps, max_families=16):
+        nf = C.c_int32(0)
+        launches = np.zeros(max_families, np.int64); ms = np.zeros(max_families)
+        names = C.create_string_buffer(1024)
+        _check(lib().gd_profile_kernels(self.h, int(nsteps), max_families, C.byref(nf),
+                                        launches.ctypes.data_as(C.POINTER(C.c_int64)), _dp(ms),
+                                        names, 1024), self.h)
+        fam = names.value.decode().split(",")[: nf.value]
+        return {f: (int(launches[k]), float(ms[k])) for k, f in enumerate(fam)}
+
+    def device_bytes(self):
+        return int(lib().gd_device_bytes(self.h))
+
+
+def selftest_mfma(A, B, Af, Bf):
+    D = np.zeros((16, 16)); Df = np.zeros((16, 16), np.float32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    A = np.ascontiguousarray(A, np.float64); B = np.ascontiguousarray(B, np.float64)
+    Af = np.ascontiguousarray(Af, np.float32); Bf = np.ascontiguousarray(Bf, np.float32)
+    _check(lib().gd_selftest_mfma(_dp(A), _dp(B), _dp(D), fp(Af), fp(Bf), fp(Df)))
+    return D, Df

@@ -1,0 +1,767 @@
+// gd_api.hip -- C ABI (include/gamil_dycore.h): handle set-up, device memory,
+// the per-step launch schedule (captured once into a hipGraph) and the
+// operator seams used by the Fortran facade and the tests.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "gamil_dycore.h"
+#include "gd_device.hpp"
+#include "gd_geom.hpp"
+#include "gd_launch.hpp"
+
+using namespace gd;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct ProfEntry {
+  std::string family;
+  hipEvent_t e0, e1;
+};
+
+}  // namespace
+
+struct gd_handle {
+  gd_config cfg{};
+  std::vector<int32_t> cutoff;
+  Geometry g;
+  int nmem = 1, prec = 0, N = 0, M = 0, P = 0, cts = 8;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+  std::vector<Member> hm;
+  Member *dm = nullptr;
+  RowCoef *dcoef = nullptr;
+  std::vector<StageSlot> slots;
+  StageSlot *dslots = nullptr;
+  std::vector<FiltTile> tiles;
+  FiltTile *dtiles = nullptr;
+  int kpad = 64, nchunk1 = 1, nchunk2 = 1;
+  double *dfcoef = nullptr, *dfs2 = nullptr, *dftab = nullptr;
+  uint32_t *ddone = nullptr;
+  int32_t *dstep = nullptr;
+  double *ddiag = nullptr;  // [nmem][cap][3]
+  int diag_cap = 512;
+  hipGraphExec_t graph = nullptr;
+  bool graph_failed = false;
+  std::string err;
+  std::vector<ProfEntry> *prof = nullptr;
+};
+
+namespace {
+
+int fail(gd_handle *h, int code, const std::string &msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+int hip_fail(gd_handle *h, hipError_t e, const char *what) {
+  return fail(h, GD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define GD_HIP(h, call)                                  \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return hip_fail((h), e_, #call); \
+  } while (0)
+
+void *dalloc(gd_handle *h, size_t bytes, hipError_t &e) {
+  void *p = nullptr;
+  e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return nullptr;
+  h->allocs.push_back(p);
+  h->bytes += (int64_t)bytes;
+  return p;
+}
+
+Args base_args(const gd_handle *h) {
+  Args a{};
+  a.mem = h->dm;
+  a.coef = h->dcoef;
+  a.slots = h->dslots;
+  a.tiles = h->dtiles;
+  a.fcoef = h->dfcoef;
+  a.fs2 = h->dfs2;
+  a.ftab = h->dftab;
+  a.done = h->ddone;
+  a.step_ctr = h->dstep;
+  a.N = h->N;
+  a.M = h->M;
+  a.pitch = h->P;
+  a.nmem = h->nmem;
+  a.m0 = 0;
+  a.ntiles = h->nmem * h->M;
+  a.nslots = (int)h->slots.size();
+  a.nftiles = (int)h->tiles.size();
+  a.kpad = h->kpad;
+  a.nchunk1 = h->nchunk1;
+  a.nchunk2 = h->nchunk2;
+  a.nchunk_row = h->nchunk2;
+  a.upwind = h->cfg.uv_adv_scheme == GD_ADV_UPWIND;
+  a.beta_lon = h->cfg.uv_adv_upwind_lon_beta;
+  a.beta_lat = h->cfg.uv_adv_upwind_lat_beta;
+  a.qcon = h->cfg.qcon_modified != 0;
+  a.diag_cap = h->diag_cap;
+  a.dlon = h->g.dlon;
+  a.dlat = h->g.dlat;
+  return a;
+}
+
+// Runs one launch, bracketed by events when profiling.
+template <typename F>
+hipError_t run(gd_handle *h, const char *family, F &&f) {
+  if (!h->prof) return f();
+  ProfEntry pe;
+  pe.family = family;
+  hipEventCreate(&pe.e0);
+  hipEventCreate(&pe.e1);
+  hipEventRecord(pe.e0, h->stream);
+  hipError_t e = f();
+  hipEventRecord(pe.e1, h->stream);
+  h->prof->push_back(pe);
+  return e;
+}
+
+hipError_t enqueue_filter(gd_handle *h, Args a) {
+  if (h->tiles.empty()) return hipSuccess;
+  a.ntiles = (int)h->tiles.size() * h->nchunk1;
+  hipError_t e = run(h, "filter_coef", [&] { return launch_filter_coef(a, h->prec, h->stream); });
+  if (e != hipSuccess) return e;
+  a.ntiles = (int)h->tiles.size() * h->nchunk2;
+  return run(h, "filter_apply", [&] { return launch_filter_apply(a, h->prec, h->stream); });
+}
+
+// predict_correct(dt, s0 -> s_out, pass) (dycore_mod.F90:754-792)
+hipError_t enqueue_call(gd_handle *h, double dt, int pass, int s0, int s_out, int qcon) {
+  Args a = base_args(h);
+  a.pass = pass;
+  a.qcon = qcon;
+  a.dt = dt;
+  a.hdt = dt * 0.5;
+  hipError_t e;
+  // first predictor: T = L(S0), S1 = S0 + dt/2 T
+  a.phase = PH_A; a.s_in = s0; a.s0 = s0; a.s_out = SP1;
+  e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, s0 == SX, h->stream); });
+  if (e != hipSuccess) return e;
+  if ((e = enqueue_filter(h, a)) != hipSuccess) return e;
+  // second predictor: T = L(S1), S2 = S0 + dt/2 T, keep T
+  a.phase = PH_B; a.s_in = SP1; a.s_out = SP2;
+  e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, false, h->stream); });
+  if (e != hipSuccess) return e;
+  if ((e = enqueue_filter(h, a)) != hipSuccess) return e;
+  // corrector tendency T = L(S2) and the beta inner-product partials
+  a.phase = PH_C; a.s_in = SP2;
+  e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, false, h->stream); });
+  if (e != hipSuccess) return e;
+  if ((e = enqueue_filter(h, a)) != hipSuccess) return e;
+  a.s_out = s_out;
+  return run(h, "correct", [&] { return launch_correct(a, h->prec, h->stream); });
+}
+
+hipError_t enqueue_diffusion(gd_handle *h, int s_in, bool in_full, double dt) {
+  Args a = base_args(h);
+  a.phase = PH_D;
+  a.s_in = s_in;
+  a.in_full = in_full;
+  const int norder = h->cfg.diffusion_order / 2;
+  const int sign = ((norder + 1) % 2 == 0) ? 1 : -1;  // (-1)**(order/2+1)
+  a.kdiff = sign * dt * h->cfg.diffusion_coef;
+  hipError_t e = run(h, "diffuse", [&] { return launch_diffuse(a, h->prec, h->stream); });
+  if (e != hipSuccess) return e;
+  return enqueue_filter(h, a);
+}
+
+hipError_t enqueue_finish(gd_handle *h, int mode, int s_in, int m0, int nm) {
+  Args a = base_args(h);
+  a.s_in = s_in;
+  a.m0 = m0;
+  a.ntiles = nm * h->M;
+  return run(h, "finish", [&] { return launch_finish(a, mode, h->stream); });
+}
+
+// time_integrate (dycore_mod.F90:654-669) + diag_run; leaves the new state in X.
+hipError_t enqueue_step(gd_handle *h) {
+  const gd_config &c = h->cfg;
+  const double dt = c.time_step_size;
+  int last;
+  hipError_t e;
+  if (c.split_scheme == GD_SPLIT_CSP2) {
+    // csp2_splitting (dycore_mod.F90:671-687)
+    const double fast_dt = dt / c.subcycles;
+    if ((e = enqueue_call(h, 0.5 * dt, GD_PASS_SLOW, SX, SY, c.qcon_modified)) != hipSuccess) return e;
+    int t1 = SY, t2 = SZ;
+    for (int k = 0; k < c.subcycles; ++k) {
+      if ((e = enqueue_call(h, fast_dt, GD_PASS_FAST, t1, t2, c.qcon_modified)) != hipSuccess) return e;
+      std::swap(t1, t2);
+    }
+    if ((e = enqueue_call(h, 0.5 * dt, GD_PASS_SLOW, t1, t2, c.qcon_modified)) != hipSuccess) return e;
+    last = t2;
+  } else {
+    // no split (and csp1, which time_integrate does not dispatch)
+    if ((e = enqueue_call(h, dt, GD_PASS_ALL, SX, SY, c.qcon_modified)) != hipSuccess) return e;
+    last = SY;
+  }
+  if (c.use_diffusion) {
+    if ((e = enqueue_diffusion(h, last, false, dt)) != hipSuccess) return e;
+    return enqueue_finish(h, 0, SX, 0, h->nmem);
+  }
+  return enqueue_finish(h, 1, last, 0, h->nmem);
+}
+
+int build_step_graph(gd_handle *h) {
+  if (h->graph || h->graph_failed) return GD_OK;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) { h->graph_failed = true; return GD_OK; }
+  hipError_t e2 = enqueue_step(h);
+  e = hipStreamEndCapture(h->stream, &g);
+  if (e2 != hipSuccess || e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    hipGetLastError();
+    h->graph_failed = true;
+    return GD_OK;
+  }
+  e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    h->graph = nullptr;
+    h->graph_failed = true;
+    hipGetLastError();
+  }
+  return GD_OK;
+}
+
+// Staging slots, filter tiles and the per-row coefficient table.
+void build_tables(gd_handle *h, std::vector<RowCoef> &rc) {
+  const Geometry &g = h->g;
+  const int M = h->M, N = h->N;
+  rc.assign(M, RowCoef{});
+  h->slots.clear();
+  for (int j = 0; j < M; ++j) {
+    RowCoef &r = rc[j];
+    r.fcos = g.full_cos[j];
+    r.f = g.full_f[j];
+    r.c = g.full_c[j];
+    r.fdlon = g.full_dlon[j];
+    r.fdlat = g.full_dlat[j];
+    r.hcos = j <= M - 2 ? g.half_cos[j] : 0.0;
+    r.hdlon = j <= M - 2 ? g.half_dlon[j] : 0.0;
+    r.hdlat = j <= M - 2 ? g.half_dlat[j] : 0.0;
+    const bool interior = j >= 1 && j <= M - 2;
+    r.k_uadv_lon = interior ? 0.25 / r.fdlon : 0.0;
+    r.k_uadv_lat = interior ? 0.25 / r.fdlat : 0.0;
+    r.k_vadv_lon = j <= M - 2 ? 0.25 / r.hdlon : 0.0;
+    r.k_vadv_lat = j <= M - 2 ? 0.25 / r.hdlat : 0.0;
+    r.c1 = interior ? g.half_cos[j - 1] / r.fcos : 0.0;
+    r.c2 = interior ? g.half_cos[j] / r.fcos : 0.0;
+    r.fdlon2 = r.fdlon * r.fdlon;
+    r.fdlat2 = r.fdlat * r.fdlat;
+    r.hdlon2 = r.hdlon * r.hdlon;
+    r.hdlat2 = r.hdlat * r.hdlat;
+    r.mass_w = r.fcos * g.dlon * g.dlat;
+    r.stage[0] = r.stage[1] = r.stage[2] = -1;
+  }
+  auto add = [&](int comp, int row, int c) {
+    StageSlot s{comp, row, kept_slots(N, c), 0};
+    rc[row].stage[comp] = (int)h->slots.size();
+    h->slots.push_back(s);
+  };
+  for (int j = 1; j <= M - 2; ++j)
+    if (g.full_flag[j]) add(CDU, j, g.full_cut[j]);
+  for (int j = 0; j <= M - 2; ++j)
+    if (g.half_flag[j]) add(CDV, j, g.half_cut[j]);
+  for (int j = 0; j < M; ++j)
+    if (g.full_flag[j]) add(CDGD, j, g.full_cut[j]);
+  // tiles: (member, slot) pairs sorted by kept count so a tile's rows share
+  // a slot range; ensemble members of one slot stay together.
+  struct Item { int kept, slot, member; };
+  std::vector<Item> items;
+  for (int s = 0; s < (int)h->slots.size(); ++s)
+    for (int m = 0; m < h->nmem; ++m) items.push_back({h->slots[s].kept, s, m});
+  std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+    return x.kept != y.kept ? x.kept < y.kept : (x.slot != y.slot ? x.slot < y.slot : x.member < y.member);
+  });
+  h->tiles.clear();
+  int kmax_range = 1;
+  for (size_t b = 0; b < items.size(); b += kFiltRows) {
+    FiltTile t{};
+    t.nrows = (int)std::min<size_t>(kFiltRows, items.size() - b);
+    int kmin = N, kmax = 0;
+    for (int r = 0; r < t.nrows; ++r) {
+      t.member[r] = items[b + r].member;
+      t.slot[r] = items[b + r].slot;
+      kmin = std::min(kmin, items[b + r].kept);
+      kmax = std::max(kmax, items[b + r].kept);
+    }
+    if (kmax <= N - kmin) {
+      t.complement = 0; t.lo = 0; t.hi = kmax;
+    } else {
+      t.complement = 1; t.lo = kmin; t.hi = N;
+    }
+    kmax_range = std::max(kmax_range, t.hi - t.lo);
+    h->tiles.push_back(t);
+  }
+  h->kpad = ((kmax_range + kChunk - 1) / kChunk) * kChunk;
+  h->nchunk1 = h->kpad / kChunk;
+  h->nchunk2 = (N + kChunk - 1) / kChunk;
+}
+
+std::string validate(const gd_config &c, int nmem) {
+  if (nmem < 1 || nmem > 4096) return "n_members must be in [1, 4096]";
+  if (!(c.time_step_size > 0)) return "time_step_size must be > 0";
+  if (c.split_scheme < 0 || c.split_scheme > 3) return "unknown split_scheme";
+  if (c.split_scheme == GD_SPLIT_CSP2 && c.subcycles < 1) return "subcycles must be >= 1";
+  if (c.uv_adv_scheme < 0 || c.uv_adv_scheme > 2) return "unknown uv_adv_scheme";
+  if (c.precision < 0 || c.precision > 2) return "unknown precision";
+  return "";
+}
+
+int check_member(gd_handle *h, int32_t member) {
+  if (!h) return GD_ERR_INVALID_ARG;
+  if (member < 0 || member >= h->nmem) return fail(h, GD_ERR_INVALID_ARG, "member out of range");
+  return GD_OK;
+}
+
+// host rows (ld apart) <-> device rows (pitch apart)
+hipError_t h2d_rows(gd_handle *h, double *dst, const double *src, int rows, int64_t ld) {
+  return hipMemcpy2DAsync(dst, (size_t)h->P * 8, src, (size_t)ld * 8, (size_t)h->N * 8, rows,
+                          hipMemcpyHostToDevice, h->stream);
+}
+hipError_t d2h_rows(gd_handle *h, double *dst, const double *src, int rows, int64_t ld) {
+  return hipMemcpy2DAsync(dst, (size_t)ld * 8, src, (size_t)h->P * 8, (size_t)h->N * 8, rows,
+                          hipMemcpyDeviceToHost, h->stream);
+}
+
+int read_tend(gd_handle *h, const Member &mb, int comp, double *out, int rows, int64_t ld) {
+  if (!out) return GD_OK;
+  std::vector<char> buf((size_t)rows * h->P * (comp == CDGD ? 8 : h->cts));
+  GD_HIP(h, hipMemcpyAsync(buf.data(), mb.tc[comp], buf.size(), hipMemcpyDeviceToHost, h->stream));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  for (int j = 0; j < rows; ++j)
+    for (int i = 0; i < h->N; ++i) {
+      const size_t k = (size_t)j * h->P + i;
+      double v;
+      if (comp == CDGD || h->cts == 8)
+        v = ((const double *)buf.data())[k];
+      else
+        v = (double)((const float *)buf.data())[k];
+      out[(size_t)j * ld + i] = v;
+    }
+  return GD_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *gd_last_error(const gd_handle *h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
+  if (!cfg || !out) return fail(nullptr, GD_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  std::string v = validate(*cfg, n_members);
+  if (!v.empty()) return fail(nullptr, GD_ERR_INVALID_ARG, v);
+  if (cfg->split_scheme == GD_SPLIT_ISP)
+    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "split_scheme 'isp' is not provided on the device path");
+  if (cfg->uv_adv_scheme == GD_ADV_WENO)
+    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "uv_adv_scheme 'weno' is not provided on the device path");
+  if (cfg->use_diffusion && cfg->diffusion_order != 2)
+    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "only diffusion_order = 2 is provided on the device path");
+  gd_handle *h = new gd_handle();
+  h->cfg = *cfg;
+  h->cutoff.assign(cfg->cutoff, cfg->cutoff + std::max(0, cfg->n_cutoff));
+  h->cfg.cutoff = h->cutoff.data();
+  h->nmem = n_members;
+  h->prec = cfg->precision == GD_PREC_FP64 ? 0 : 1;
+  h->cts = h->prec == 0 ? 8 : 4;
+  h->N = cfg->num_lon;
+  h->M = cfg->num_lat;
+  h->P = ((h->N + 31) / 32) * 32;
+  h->device = cfg->device;
+  std::string ge = build_geometry(h->N, h->M, cfg->use_zonal_tend_filter != 0, h->cutoff.data(),
+                                  (int)h->cutoff.size(), h->g);
+  if (!ge.empty()) {
+    delete h;
+    return fail(nullptr, GD_ERR_INVALID_ARG, ge);
+  }
+  hipError_t e = hipSetDevice(h->device);
+  if (e != hipSuccess) {
+    std::string m = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    delete h;
+    return fail(nullptr, GD_ERR_HIP, m);
+  }
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, h->device);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(nullptr, GD_ERR_HIP, "hipGetDeviceProperties failed");
+  }
+  std::vector<RowCoef> rc;
+  build_tables(h, rc);
+  const size_t lds_need = std::max({tend_lds_bytes(h->N, h->cts), diffuse_lds_bytes(h->N),
+                                    filt_coef_lds_bytes(h->N, h->cts),
+                                    filt_apply_lds_bytes(h->N, h->kpad, h->cts)});
+  if (lds_need > prop.sharedMemPerBlock) {
+    delete h;
+    return fail(nullptr, GD_ERR_INVALID_ARG, "num_lon too large for the LDS of this device");
+  }
+  configure_kernels((int)prop.sharedMemPerBlock);
+  hipGetLastError();
+  e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(nullptr, GD_ERR_HIP, "hipStreamCreate failed");
+  }
+  // ---- device memory: one arena per member ----
+  const size_t fld = (size_t)h->M * h->P;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t nslots = h->slots.size();
+  const size_t per_member = al(fld * 8) * (NSLOT * NFLD + 1 + 2 + 2) +  // states, ghs, tb/tc dgd, div/vor
+                            al(fld * h->cts) * 4 +                        // tb/tc du, dv
+                            al((size_t)h->M * 3 * h->nchunk2 * 2 * 8) + al((size_t)h->M * 4 * 8) +
+                            al(std::max<size_t>(nslots, 1) * h->P * h->cts) +
+                            al(std::max<size_t>(nslots, 1) * 8) + al(8);
+  h->hm.resize(h->nmem);
+  for (int m = 0; m < h->nmem; ++m) {
+    char *base = (char *)dalloc(h, per_member, e);
+    if (!base) { gd_destroy(h); return fail(nullptr, GD_ERR_HIP, "device allocation failed"); }
+    e = hipMemsetAsync(base, 0, per_member, h->stream);
+    Member &mb = h->hm[m];
+    char *p = base;
+    auto take = [&](size_t b) { char *q = p; p += al(b); return q; };
+    for (int s = 0; s < NSLOT; ++s)
+      for (int f = 0; f < NFLD; ++f) mb.st[s][f] = (double *)take(fld * 8);
+    mb.ghs = (double *)take(fld * 8);
+    mb.tb[CDU] = take(fld * h->cts); mb.tb[CDV] = take(fld * h->cts); mb.tb[CDGD] = take(fld * 8);
+    mb.tc[CDU] = take(fld * h->cts); mb.tc[CDV] = take(fld * h->cts); mb.tc[CDGD] = take(fld * 8);
+    mb.div = (double *)take(fld * 8);
+    mb.vor = (double *)take(fld * 8);
+    mb.part = (double *)take((size_t)h->M * 3 * h->nchunk2 * 2 * 8);
+    mb.dpart = (double *)take((size_t)h->M * 4 * 8);
+    mb.sx = take(std::max<size_t>(nslots, 1) * h->P * h->cts);
+    mb.s1 = (double *)take(std::max<size_t>(nslots, 1) * 8);
+    mb.beta = (double *)take(8);
+  }
+  h->ddiag = (double *)dalloc(h, (size_t)h->nmem * h->diag_cap * 3 * 8, e);
+  for (int m = 0; m < h->nmem && h->ddiag; ++m) h->hm[m].diag_out = h->ddiag + (size_t)m * h->diag_cap * 3;
+  h->dm = (Member *)dalloc(h, sizeof(Member) * h->nmem, e);
+  h->dcoef = (RowCoef *)dalloc(h, sizeof(RowCoef) * h->M, e);
+  h->dslots = (StageSlot *)dalloc(h, sizeof(StageSlot) * std::max<size_t>(nslots, 1), e);
+  h->dtiles = (FiltTile *)dalloc(h, sizeof(FiltTile) * std::max<size_t>(h->tiles.size(), 1), e);
+  h->dfcoef = (double *)dalloc(h, std::max<size_t>(h->tiles.size(), 1) * kFiltRows * h->kpad * 8, e);
+  h->dfs2 = (double *)dalloc(h, std::max<size_t>(h->tiles.size(), 1) * h->nchunk1 * kFiltRows * 8, e);
+  h->dftab = (double *)dalloc(h, (size_t)2 * h->N * 8, e);
+  h->ddone = (uint32_t *)dalloc(h, sizeof(uint32_t) * h->nmem, e);
+  h->dstep = (int32_t *)dalloc(h, sizeof(int32_t) * h->nmem, e);
+  if (!h->ddiag || !h->dm || !h->dcoef || !h->dslots || !h->dtiles || !h->dfcoef || !h->dfs2 ||
+      !h->dftab || !h->ddone || !h->dstep) {
+    gd_destroy(h);
+    return fail(nullptr, GD_ERR_HIP, "device allocation failed");
+  }
+  std::vector<double> tab(2 * h->N);
+  for (int q = 0; q < h->N; ++q) {
+    const double ang = 2.0 * kPi * (double)q / (double)h->N;
+    tab[q] = std::cos(ang);
+    tab[h->N + q] = std::sin(ang);
+  }
+  hipMemcpyAsync(h->dm, h->hm.data(), sizeof(Member) * h->nmem, hipMemcpyHostToDevice, h->stream);
+  hipMemcpyAsync(h->dcoef, rc.data(), sizeof(RowCoef) * h->M, hipMemcpyHostToDevice, h->stream);
+  if (nslots) hipMemcpyAsync(h->dslots, h->slots.data(), sizeof(StageSlot) * nslots, hipMemcpyHostToDevice, h->stream);
+  if (!h->tiles.empty())
+    hipMemcpyAsync(h->dtiles, h->tiles.data(), sizeof(FiltTile) * h->tiles.size(), hipMemcpyHostToDevice, h->stream);
+  hipMemcpyAsync(h->dftab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, h->stream);
+  hipMemsetAsync(h->ddone, 0, sizeof(uint32_t) * h->nmem, h->stream);
+  hipMemsetAsync(h->dstep, 0, sizeof(int32_t) * h->nmem, h->stream);
+  e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    std::string m = std::string("set-up: ") + hipGetErrorString(e);
+    gd_destroy(h);
+    return fail(nullptr, GD_ERR_HIP, m);
+  }
+  *out = h;
+  return GD_OK;
+}
+
+int gd_destroy(gd_handle *h) {
+  if (!h) return GD_OK;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  for (void *p : h->allocs) hipFree(p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return GD_OK;
+}
+
+int gd_set_state(gd_handle *h, int32_t member, const double *u, const double *v, const double *gdp,
+                 const double *ghs, int64_t ld) {
+  int rc = check_member(h, member);
+  if (rc) return rc;
+  if (!u || !v || !gdp || !ghs || ld < h->N) return fail(h, GD_ERR_INVALID_ARG, "bad state arguments");
+  const int N = h->N, M = h->M;
+  for (int i = 0; i < N; ++i)
+    if (u[i] != 0.0 || u[(size_t)(M - 1) * ld + i] != 0.0)
+      return fail(h, GD_ERR_INVALID_ARG, "u must be zero on the pole rows");
+  for (int j = 0; j < M; ++j)
+    for (int i = 0; i < N; ++i)
+      if (!(gdp[(size_t)j * ld + i] > 0.0)) return fail(h, GD_ERR_INVALID_ARG, "gd must be positive");
+  hipSetDevice(h->device);
+  Member &mb = h->hm[member];
+  GD_HIP(h, h2d_rows(h, mb.st[SX][Fu], u, M, ld));
+  GD_HIP(h, h2d_rows(h, mb.st[SX][Fv], v, M - 1, ld));
+  GD_HIP(h, h2d_rows(h, mb.st[SX][FGD], gdp, M, ld));
+  GD_HIP(h, h2d_rows(h, mb.ghs, ghs, M, ld));
+  // iap_transform(state(old)) + diag (dycore_mod.F90:123-125)
+  GD_HIP(h, enqueue_finish(h, 0, SX, member, 1));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  return GD_OK;
+}
+
+int gd_diag(gd_handle *h, double *mass, double *energy) {
+  if (!h) return GD_ERR_INVALID_ARG;
+  hipSetDevice(h->device);
+  GD_HIP(h, hipMemsetAsync(h->dstep, 0, sizeof(int32_t) * h->nmem, h->stream));
+  GD_HIP(h, enqueue_finish(h, 2, SX, 0, h->nmem));
+  std::vector<double> buf((size_t)h->nmem * h->diag_cap * 3);
+  GD_HIP(h, hipMemcpyAsync(buf.data(), h->ddiag, buf.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  for (int m = 0; m < h->nmem; ++m) {
+    if (mass) mass[m] = buf[(size_t)m * h->diag_cap * 3 + 0];
+    if (energy) energy[m] = buf[(size_t)m * h->diag_cap * 3 + 1];
+  }
+  return GD_OK;
+}
+
+int gd_step(gd_handle *h, int32_t nsteps, double *mass, double *energy, double *beta, int32_t *nan_step) {
+  if (!h || nsteps < 0) return GD_ERR_INVALID_ARG;
+  hipSetDevice(h->device);
+  if (nan_step) *nan_step = 0;
+  if (!h->prof) build_step_graph(h);
+  std::vector<double> buf((size_t)h->nmem * h->diag_cap * 3);
+  int first_nan = 0;
+  for (int done = 0; done < nsteps;) {
+    const int n = std::min(h->diag_cap, nsteps - done);
+    GD_HIP(h, hipMemsetAsync(h->dstep, 0, sizeof(int32_t) * h->nmem, h->stream));
+    for (int k = 0; k < n; ++k) {
+      if (h->graph && !h->prof) {
+        GD_HIP(h, hipGraphLaunch(h->graph, h->stream));
+      } else {
+        GD_HIP(h, enqueue_step(h));
+      }
+    }
+    GD_HIP(h, hipMemcpyAsync(buf.data(), h->ddiag, buf.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    GD_HIP(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < n; ++k)
+      for (int m = 0; m < h->nmem; ++m) {
+        const double *d = &buf[((size_t)m * h->diag_cap + k) * 3];
+        const size_t o = (size_t)(done + k) * h->nmem + m;
+        if (mass) mass[o] = d[0];
+        if (energy) energy[o] = d[1];
+        if (beta) beta[o] = d[2];
+        if (!first_nan && (std::isnan(d[0]) || std::isnan(d[1]))) first_nan = done + k + 1;
+      }
+    done += n;
+  }
+  if (first_nan) {
+    if (nan_step) *nan_step = first_nan;
+    return fail(h, GD_ERR_NAN, "Total mass or energy is NaN at step " + std::to_string(first_nan));
+  }
+  return GD_OK;
+}
+
+int gd_get_state(gd_handle *h, int32_t member, double *u, double *v, double *gdp, int64_t ld) {
+  int rc = check_member(h, member);
+  if (rc) return rc;
+  if (ld < h->N) return fail(h, GD_ERR_INVALID_ARG, "ld < num_lon");
+  hipSetDevice(h->device);
+  const Member &mb = h->hm[member];
+  if (u) GD_HIP(h, d2h_rows(h, u, mb.st[SX][Fu], h->M, ld));
+  if (v) GD_HIP(h, d2h_rows(h, v, mb.st[SX][Fv], h->M - 1, ld));
+  if (gdp) GD_HIP(h, d2h_rows(h, gdp, mb.st[SX][FGD], h->M, ld));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  return GD_OK;
+}
+
+int gd_get_diag_fields(gd_handle *h, int32_t member, double *div, double *vor, int64_t ld) {
+  int rc = check_member(h, member);
+  if (rc) return rc;
+  if (ld < h->N) return fail(h, GD_ERR_INVALID_ARG, "ld < num_lon");
+  hipSetDevice(h->device);
+  Args a = base_args(h);
+  a.m0 = member;
+  a.ntiles = h->M;
+  GD_HIP(h, launch_divvor(a, h->stream));
+  const Member &mb = h->hm[member];
+  if (div) GD_HIP(h, d2h_rows(h, div, mb.div, h->M, ld));
+  if (vor) GD_HIP(h, d2h_rows(h, vor, mb.vor, h->M - 1, ld));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  return GD_OK;
+}
+
+int gd_space_operators(gd_handle *h, int32_t member, int32_t pass, double *du, double *dv, double *dgd,
+                       int64_t ld) {
+  int rc = check_member(h, member);
+  if (rc) return rc;
+  if (pass < 0 || pass > 2 || ld < h->N) return fail(h, GD_ERR_INVALID_ARG, "bad pass or ld");
+  hipSetDevice(h->device);
+  Args a = base_args(h);
+  a.pass = pass;
+  a.phase = PH_T;
+  a.s_in = SX;
+  a.s0 = SX;
+  GD_HIP(h, launch_tend(a, h->prec, pass, true, h->stream));
+  GD_HIP(h, enqueue_filter(h, a));
+  const Member &mb = h->hm[member];
+  if ((rc = read_tend(h, mb, CDU, du, h->M, ld))) return rc;
+  if ((rc = read_tend(h, mb, CDV, dv, h->M - 1, ld))) return rc;
+  if ((rc = read_tend(h, mb, CDGD, dgd, h->M, ld))) return rc;
+  return GD_OK;
+}
+
+int gd_predict_correct(gd_handle *h, double dt, int32_t pass, int32_t qcon, double *beta) {
+  if (!h || pass < 0 || pass > 2) return GD_ERR_INVALID_ARG;
+  hipSetDevice(h->device);
+  GD_HIP(h, enqueue_call(h, dt, pass, SX, SY, qcon));
+  GD_HIP(h, enqueue_finish(h, 1, SY, 0, h->nmem));
+  if (beta)
+    for (int m = 0; m < h->nmem; ++m)
+      GD_HIP(h, hipMemcpyAsync(beta + m, h->hm[m].beta, 8, hipMemcpyDeviceToHost, h->stream));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  return GD_OK;
+}
+
+int gd_ordinary_diffusion(gd_handle *h, double dt) {
+  if (!h) return GD_ERR_INVALID_ARG;
+  if (h->cfg.diffusion_order != 2) return fail(h, GD_ERR_NOT_SUPPORTED, "diffusion_order != 2");
+  hipSetDevice(h->device);
+  const size_t fb = (size_t)h->M * h->P * 8;
+  for (int m = 0; m < h->nmem; ++m)
+    for (int f = 0; f < NFLD; ++f)
+      GD_HIP(h, hipMemcpyAsync(h->hm[m].st[SY][f], h->hm[m].st[SX][f], fb, hipMemcpyDeviceToDevice, h->stream));
+  GD_HIP(h, enqueue_diffusion(h, SY, true, dt));
+  GD_HIP(h, enqueue_finish(h, 0, SX, 0, h->nmem));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  return GD_OK;
+}
+
+int gd_filter_rows(gd_handle *h, int32_t half, double *x, int64_t ld) {
+  if (!h || !x || ld < h->N) return GD_ERR_INVALID_ARG;
+  hipSetDevice(h->device);
+  const int comp = half ? CDV : CDGD;
+  const Member &mb = h->hm[0];
+  const size_t rowb = (size_t)h->P * h->cts;
+  std::vector<char> row(rowb);
+  for (size_t s = 0; s < h->slots.size(); ++s) {
+    if (h->slots[s].comp != comp) continue;
+    const double *src = x + (size_t)h->slots[s].row * ld;
+    for (int i = 0; i < h->N; ++i) {
+      if (h->cts == 8) ((double *)row.data())[i] = src[i];
+      else ((float *)row.data())[i] = (float)src[i];
+    }
+    GD_HIP(h, hipMemcpy((char *)mb.sx + s * rowb, row.data(), rowb, hipMemcpyHostToDevice));
+  }
+  Args a = base_args(h);
+  a.phase = PH_P;
+  GD_HIP(h, enqueue_filter(h, a));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  for (size_t s = 0; s < h->slots.size(); ++s) {
+    if (h->slots[s].comp != comp) continue;
+    GD_HIP(h, hipMemcpy(row.data(), (char *)mb.sx + s * rowb, rowb, hipMemcpyDeviceToHost));
+    double *dst = x + (size_t)h->slots[s].row * ld;
+    for (int i = 0; i < h->N; ++i)
+      dst[i] = h->cts == 8 ? ((double *)row.data())[i] : (double)((float *)row.data())[i];
+  }
+  return GD_OK;
+}
+
+int gd_filter_table(const gd_handle *h, int32_t *full_flag, int32_t *half_flag, int32_t *full_cut,
+                    int32_t *half_cut) {
+  if (!h) return GD_ERR_INVALID_ARG;
+  for (int j = 0; j < h->M; ++j) {
+    if (full_flag) full_flag[j] = h->g.full_flag[j];
+    if (full_cut) full_cut[j] = h->g.full_cut[j];
+  }
+  for (int j = 0; j < h->M - 1; ++j) {
+    if (half_flag) half_flag[j] = h->g.half_flag[j];
+    if (half_cut) half_cut[j] = h->g.half_cut[j];
+  }
+  return GD_OK;
+}
+
+int gd_profile_kernels(gd_handle *h, int32_t nsteps, int32_t max_families, int32_t *n_families,
+                       int64_t *launches, double *ms, char *names, int32_t name_cap) {
+  if (!h || nsteps < 1) return GD_ERR_INVALID_ARG;
+  hipSetDevice(h->device);
+  std::vector<ProfEntry> entries;
+  h->prof = &entries;
+  int rc = gd_step(h, nsteps, nullptr, nullptr, nullptr, nullptr);
+  h->prof = nullptr;
+  hipStreamSynchronize(h->stream);
+  std::vector<std::string> order;
+  std::map<std::string, std::pair<int64_t, double>> acc;
+  for (auto &pe : entries) {
+    float t = 0;
+    hipEventElapsedTime(&t, pe.e0, pe.e1);
+    if (!acc.count(pe.family)) order.push_back(pe.family);
+    acc[pe.family].first += 1;
+    acc[pe.family].second += t;
+    hipEventDestroy(pe.e0);
+    hipEventDestroy(pe.e1);
+  }
+  if (rc && rc != GD_ERR_NAN) return rc;
+  int nf = std::min<int>((int)order.size(), max_families);
+  std::string joined;
+  for (int k = 0; k < nf; ++k) {
+    if (launches) launches[k] = acc[order[k]].first;
+    if (ms) ms[k] = acc[order[k]].second;
+    if (k) joined += ",";
+    joined += order[k];
+  }
+  if (n_families) *n_families = nf;
+  if (names && name_cap > 0) {
+    std::snprintf(names, name_cap, "%s", joined.c_str());
+  }
+  return GD_OK;
+}
+
+int64_t gd_device_bytes(const gd_handle *h) { return h ? h->bytes : 0; }
+
+// Device check of the MFMA operand layout the filter kernels rely on:
+// D = A B with A 16x4 and B 4x16 (row-major), FP64 and FP32 variants.
+int gd_selftest_mfma(const double *A, const double *B, double *D, const float *Af, const float *Bf,
+                     float *Df) {
+  double *dA, *dB, *dD;
+  float *fA, *fB, *fD;
+  if (hipMalloc(&dA, 64 * 8) || hipMalloc(&dB, 64 * 8) || hipMalloc(&dD, 256 * 8) ||
+      hipMalloc(&fA, 64 * 4) || hipMalloc(&fB, 64 * 4) || hipMalloc(&fD, 256 * 4))
+    return GD_ERR_HIP;
+  hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice);
+  hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice);
+  hipMemcpy(fA, Af, 64 * 4, hipMemcpyHostToDevice);
+  hipMemcpy(fB, Bf, 64 * 4, hipMemcpyHostToDevice);
+  hipError_t e = launch_mfma_layout(dA, dB, dD, fA, fB, fD, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  hipMemcpy(D, dD, 256 * 8, hipMemcpyDeviceToHost);
+  hipMemcpy(Df, fD, 256 * 4, hipMemcpyDeviceToHost);
+  hipFree(dA); hipFree(dB); hipFree(dD); hipFree(fA); hipFree(fB); hipFree(fD);
+  return e == hipSuccess ? GD_OK : GD_ERR_HIP;
+}
+
+}  // extern "C"

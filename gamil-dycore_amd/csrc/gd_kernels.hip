@@ -1,0 +1,979 @@
+// gd_kernels.hip -- CDNA4 (gfx950) kernels of the barotropic time step.
+//
+// Row kernels: one workgroup (256 threads, 4 wave64) per (member, latitude
+// row); tiles are remapped so each XCD owns a contiguous band of rows and
+// the +-1 row stencil reuse stays inside that XCD's L2.  Derived fields
+// (G = sqrt(gd), u, v) of the rows a tile needs are built once in LDS.
+//
+// Filter kernels: the polar zonal filter (filter_mod.F90) as a batched
+// truncated real-DFT product on MFMA, C = X B^T (coefficients) then
+// Y = C' B (reconstruction), over tiles of 8 staged rows.
+//
+// FP64 mode reproduces the reference's expressions operation by operation
+// (the library is built with -ffp-contract=off); mixed mode (CT = float)
+// evaluates the momentum stencils in FP32 and keeps the state update, mass
+// flux divergence, pole sums, geopotential differences and all inner
+// products in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gd_device.hpp"
+#include "gd_geom.hpp"
+#include "gd_launch.hpp"
+
+namespace gd {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+  const int b = blockIdx.x, nb = gridDim.x;  // nb is a multiple of kXcd
+  const int per = nb / kXcd;
+  const int t = (b % kXcd) * per + b / kXcd;
+  return t < ntiles ? t : -1;
+}
+
+// Deterministic workgroup sum (fixed butterfly + fixed cross-wave order);
+// every thread returns the same value.
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = red[0];
+  for (int k = 1; k < kBlock / 64; ++k) s += red[k];
+  return s;
+}
+
+// Sum over the 16 lanes of a lane group (lane % 16 varies), deterministic.
+__device__ __forceinline__ double group16_sum(double v) {
+  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double fld_base(const Member &mb, int slot, int comp, size_t o) {
+  return mb.st[slot][comp == CDU ? FU : (comp == CDV ? FV : FGD)][o];
+}
+
+template <typename CT>
+__device__ __forceinline__ double tend_load(const Member &mb, bool tb, int comp, size_t o) {
+  void *p = tb ? mb.tb[comp] : mb.tc[comp];
+  if (comp == CDGD) return ((const double *)p)[o];
+  return (double)((const CT *)p)[o];
+}
+
+template <typename CT>
+__device__ __forceinline__ void tend_store(const Member &mb, bool tb, int comp, size_t o, double v) {
+  void *p = tb ? mb.tb[comp] : mb.tc[comp];
+  if (comp == CDGD)
+    ((double *)p)[o] = v;
+  else
+    ((CT *)p)[o] = (CT)v;
+}
+
+// Epilogue of one tendency value of a predict/correct phase (not diffusion).
+// Returns the (Tb*T, T*T) products of phase C for the beta partials.
+template <typename CT>
+__device__ __forceinline__ void phase_epilogue(const Args &a, const Member &mb, int comp, int row,
+                                               int i, double T, double w, double &p1, double &p2) {
+  const size_t o = (size_t)row * a.pitch + i;
+  const int f = comp == CDU ? FU : (comp == CDV ? FV : FGD);
+  if (a.phase == PH_A || a.phase == PH_B) {
+    mb.st[a.s_out][f][o] = mb.st[a.s0][f][o] + a.hdt * T;
+    if (a.phase == PH_B) tend_store<CT>(mb, true, comp, o, T);
+  } else {
+    tend_store<CT>(mb, false, comp, o, T);
+    if (a.phase == PH_C) {
+      double tbv = tend_load<CT>(mb, true, comp, o);
+      p1 += tbv * T * w;
+      p2 += T * T * w;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tendency (space_operators, dycore_mod.F90:184-598) + predictor epilogue
+// ---------------------------------------------------------------------------
+
+template <typename CT, int PASS, bool IN_FULL>
+__global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const int N = a.N, M = a.M, P = a.pitch;
+  const int m = a.m0 + t / M, j = t % M;
+  const Member &mb = a.mem[m];
+  extern __shared__ __align__(16) unsigned char smem[];
+  double *sG = (double *)smem;        // G rows j-1 .. j+2
+  CT *su = (CT *)(sG + 4 * N);        // u rows j, j+1
+  CT *sv = su + 2 * N;                // v half rows j-1, j, j+1
+  double *red = (double *)(smem + tend_lds_bytes(N, sizeof(CT)) - 16 * sizeof(double));
+  const int tid = threadIdx.x;
+  const double *gd = mb.st[a.s_in][FGD];
+  const double *Ug = mb.st[a.s_in][FU];
+  const double *Vg = mb.st[a.s_in][FV];
+  const double *ghs = mb.ghs;
+
+  for (int r = 0; r < 4; ++r) {
+    const int row = j - 1 + r;
+    if (row < 0 || row > M - 1) continue;
+    for (int i = tid; i < N; i += kBlock) sG[r * N + i] = sqrt(gd[(size_t)row * P + i]);
+  }
+  __syncthreads();
+  for (int r = 0; r < 2; ++r) {
+    const int row = j + r;
+    if (row > M - 1) continue;
+    for (int i = tid; i < N; i += kBlock) {
+      CT val;
+      if (IN_FULL) {
+        val = (CT)mb.st[a.s_in][Fu][(size_t)row * P + i];
+      } else if (row == 0 || row == M - 1) {
+        val = (CT)0;
+      } else {
+        const int ip = i + 1 == N ? 0 : i + 1;
+        // update_state: u = U * 2 / (G(i) + G(i+1))  (dycore_mod.F90:636-640)
+        val = ((CT)Ug[(size_t)row * P + i] * (CT)2) / ((CT)sG[(r + 1) * N + i] + (CT)sG[(r + 1) * N + ip]);
+      }
+      su[r * N + i] = val;
+    }
+  }
+  for (int r = 0; r < 3; ++r) {
+    const int hrow = j - 1 + r;
+    const bool ok = hrow >= 0 && hrow <= M - 2;
+    for (int i = tid; i < N; i += kBlock) {
+      CT val = (CT)0;
+      if (ok) {
+        if (IN_FULL)
+          val = (CT)mb.st[a.s_in][Fv][(size_t)hrow * P + i];
+        else  // v = V * 2 / (G(j) + G(j+1))  (dycore_mod.F90:641-645)
+          val = ((CT)Vg[(size_t)hrow * P + i] * (CT)2) / ((CT)sG[r * N + i] + (CT)sG[(r + 1) * N + i]);
+      }
+      sv[r * N + i] = val;
+    }
+  }
+  __syncthreads();
+
+  const RowCoef cj = a.coef[j];
+  const bool has_u = j >= 1 && j <= M - 2;
+  const bool has_v = j <= M - 2;
+  const double hc_m = j >= 1 ? a.coef[j - 1].hcos : 0.0;
+  const double hc_0 = has_v ? cj.hcos : 0.0;
+  const double hc_p = j + 1 <= M - 2 ? a.coef[j + 1].hcos : 0.0;
+  const double f_p = j + 1 <= M - 1 ? a.coef[j + 1].f : 0.0;
+  const double c_p = j + 1 <= M - 1 ? a.coef[j + 1].c : 0.0;
+  const CT bl = (CT)a.beta_lon, bt = (CT)a.beta_lat;
+  const bool upwind = a.upwind != 0;
+
+  // pole mass flux sums (dycore_mod.F90:572-596)
+  double pole = 0.0;
+  if (PASS != 2 && (j == 0 || j == M - 1)) {
+    double acc = 0.0;
+    if (j == 0) {
+      for (int i = tid; i < N; i += kBlock) acc += (sG[N + i] + sG[2 * N + i]) * Vg[i];
+    } else {
+      for (int i = tid; i < N; i += kBlock) acc += (sG[N + i] + sG[i]) * Vg[(size_t)(M - 2) * P + i];
+    }
+    double s = block_sum(acc, red);
+    if (j == M - 1) s = -s;
+    pole = s * 2.0 / N / kRadius / a.dlat;
+  }
+
+  const int st_du = has_u ? cj.stage[CDU] : -1;
+  const int st_dv = has_v ? cj.stage[CDV] : -1;
+  const int st_dgd = cj.stage[CDGD];
+  double s1_du = 0, s1_dv = 0, s1_dgd = 0;
+  double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
+  CT *sx = (CT *)mb.sx;
+
+  for (int i = tid; i < N; i += kBlock) {
+    const int im = i == 0 ? N - 1 : i - 1, ip = i == N - 1 ? 0 : i + 1;
+    const size_t o = (size_t)j * P;
+    // ---- u equation (full row j) ----
+    double du = 0.0;
+    if (has_u) {
+      const CT u0 = su[i], um = su[im], up = su[ip];
+      CT ual = 0, uat = 0, fv = 0, upgf = 0;
+      if (PASS != 1) {
+        const CT U0 = (CT)Ug[o + i], Um = (CT)Ug[o + im], Up = (CT)Ug[o + ip];
+        const CT u1 = u0 + um, u2 = u0 + up;
+        const CT kl = (CT)cj.k_uadv_lon;
+        if (!upwind)
+          ual = kl * (u2 * Up - u1 * Um);
+        else
+          ual = kl * (u2 * (U0 + Up) - bl * fabs(u2) * (Up - U0) - u1 * (U0 + Um) +
+                      bl * fabs(u1) * (U0 - Um) - (u2 - u1) * U0);
+        const CT v1 = (sv[i] + sv[ip]) * (CT)hc_m;
+        const CT v2 = (sv[N + i] + sv[N + ip]) * (CT)hc_0;
+        const CT Ujp = (CT)Ug[o + P + i], Ujm = (CT)Ug[o - P + i];
+        const CT kt = (CT)cj.k_uadv_lat;
+        if (!upwind)
+          uat = kt * (v2 * Ujp - v1 * Ujm);
+        else
+          uat = kt * (v2 * (U0 + Ujp) - bt * fabs(v2) * (Ujp - U0) - v1 * (U0 + Ujm) +
+                      bt * fabs(v1) * (U0 - Ujm) - (v2 - v1) * U0);
+      }
+      if (PASS != 2) {
+        const CT Vm0 = (CT)Vg[o - P + i], Vm1 = (CT)Vg[o - P + ip];
+        const CT V00 = (CT)Vg[o + i], V01 = (CT)Vg[o + ip];
+        fv = (CT)0.25 * ((CT)cj.f + (CT)cj.c * u0) * ((CT)cj.c1 * (Vm0 + Vm1) + (CT)cj.c2 * (V00 + V01));
+        const double dphi = gd[o + ip] + ghs[o + ip] - gd[o + i] - ghs[o + i];
+        upgf = (CT)0.5 * ((CT)sG[N + i] + (CT)sG[N + ip]) / (CT)cj.fdlon * (CT)dphi;
+      }
+      CT d;
+      if (PASS == 0)
+        d = -ual - uat + fv - upgf;
+      else if (PASS == 1)
+        d = fv - upgf;
+      else
+        d = -ual - uat;
+      du = (double)d;
+    }
+    // ---- v equation (half row j) ----
+    double dv = 0.0;
+    if (has_v) {
+      CT val = 0, vat = 0, fu = 0, vpgf = 0;
+      if (PASS != 1) {
+        const CT u1 = su[im] + su[N + im];
+        const CT u2 = su[i] + su[N + i];
+        const CT V0 = (CT)Vg[o + i], Vm = (CT)Vg[o + im], Vp = (CT)Vg[o + ip];
+        const CT kl = (CT)cj.k_vadv_lon;
+        if (!upwind)
+          val = kl * (u2 * Vp - u1 * Vm);
+        else
+          val = kl * (u2 * (V0 + Vp) - bl * fabs(u2) * (Vp - V0) - u1 * (V0 + Vm) +
+                      bl * fabs(u1) * (V0 - Vm) - (u2 - u1) * V0);
+        const CT vj = sv[N + i];
+        const CT v1 = vj * (CT)hc_0 + sv[i] * (CT)hc_m;
+        const CT v2 = vj * (CT)hc_0 + sv[2 * N + i] * (CT)hc_p;
+        const CT Vjp = j + 1 <= M - 2 ? (CT)Vg[o + P + i] : (CT)0;
+        const CT Vjm = j >= 1 ? (CT)Vg[o - P + i] : (CT)0;
+        const CT kt = (CT)cj.k_vadv_lat;
+        if (!upwind)
+          vat = kt * (v2 * Vjp - v1 * Vjm);
+        else
+          vat = kt * (v2 * (V0 + Vjp) - bt * fabs(v2) * (Vjp - V0) - v1 * (V0 + Vjm) +
+                      bt * fabs(v1) * (V0 - Vjm) - (v2 - v1) * V0);
+      }
+      if (PASS != 2) {
+        const CT f0 = (CT)cj.f, c0 = (CT)cj.c, fp = (CT)f_p, cp = (CT)c_p;
+        const CT a1 = (f0 + c0 * su[i]) * (CT)Ug[o + i];
+        const CT a2 = (f0 + c0 * su[im]) * (CT)Ug[o + im];
+        const CT a3 = (fp + cp * su[N + i]) * (CT)Ug[o + P + i];
+        const CT a4 = (fp + cp * su[N + im]) * (CT)Ug[o + P + im];
+        fu = (CT)0.25 * (a1 + a2 + a3 + a4);
+        const double dphi = gd[o + P + i] + ghs[o + P + i] - gd[o + i] - ghs[o + i];
+        vpgf = (CT)0.5 * ((CT)sG[N + i] + (CT)sG[2 * N + i]) / (CT)cj.hdlat * (CT)cj.hcos * (CT)dphi;
+      }
+      CT d;
+      if (PASS == 0)
+        d = -val - vat - fu - vpgf;
+      else if (PASS == 1)
+        d = -fu - vpgf;
+      else
+        d = -val - vat;
+      dv = (double)d;
+    }
+    // ---- mass continuity (full row j), FP64 in every mode ----
+    double dgd = 0.0;
+    if (PASS != 2) {
+      double mdl = 0.0, mdp;
+      if (j == 0 || j == M - 1) {
+        mdp = pole;
+      } else {
+        const double Gi = sG[N + i], Gip = sG[N + ip], Gim = sG[N + im];
+        mdl = ((Gi + Gip) * Ug[o + i] - (Gi + Gim) * Ug[o + im]) * 0.5 / cj.fdlon;
+        mdp = ((Gi + sG[2 * N + i]) * Vg[o + i] * hc_0 - (Gi + sG[i]) * Vg[o - P + i] * hc_m) * 0.5 / cj.fdlat;
+      }
+      dgd = -mdl - mdp;
+    }
+    // ---- filter staging or epilogue ----
+    if (st_du >= 0) {
+      sx[(size_t)st_du * P + i] = (CT)du;
+      s1_du += du * Ug[o + i];
+    } else {
+      phase_epilogue<CT>(a, mb, CDU, j, i, du, cj.fcos, p1[0], p2[0]);
+    }
+    if (has_v) {
+      if (st_dv >= 0) {
+        sx[(size_t)st_dv * P + i] = (CT)dv;
+        s1_dv += dv * Vg[o + i];
+      } else {
+        phase_epilogue<CT>(a, mb, CDV, j, i, dv, cj.hcos, p1[1], p2[1]);
+      }
+    }
+    if (st_dgd >= 0) {
+      sx[(size_t)st_dgd * P + i] = (CT)dgd;
+      s1_dgd += dgd * (gd[o + i] + ghs[o + i]);
+    } else {
+      phase_epilogue<CT>(a, mb, CDGD, j, i, dgd, cj.fcos, p1[2], p2[2]);
+    }
+  }
+  // row reductions (uniform control flow: every branch depends on j only)
+  if (st_du >= 0) {
+    double s = block_sum(s1_du, red);
+    if (tid == 0) mb.s1[st_du] = s;
+  }
+  if (st_dv >= 0) {
+    double s = block_sum(s1_dv, red);
+    if (tid == 0) mb.s1[st_dv] = s;
+  }
+  if (st_dgd >= 0) {
+    double s = block_sum(s1_dgd, red);
+    if (tid == 0) mb.s1[st_dgd] = s;
+  }
+  if (a.phase == PH_C) {
+    for (int c = 0; c < 3; ++c) {
+      const bool staged = c == CDU ? st_du >= 0 : (c == CDV ? st_dv >= 0 : st_dgd >= 0);
+      if (staged || (c == CDV && !has_v)) continue;
+      double s1v = block_sum(p1[c], red);
+      double s2v = block_sum(p2[c], red);
+      if (tid == 0) {
+        double *pp = mb.part + ((size_t)(j * 3 + c) * a.nchunk_row) * 2;
+        pp[0] = s1v;
+        pp[1] = s2v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// corrector update (predict_correct, dycore_mod.F90:784-790)
+// ---------------------------------------------------------------------------
+
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) k_correct(const Args a) {
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const int N = a.N, M = a.M, P = a.pitch;
+  const int m = a.m0 + t / M, j = t % M;
+  const Member &mb = a.mem[m];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  // inner_product(tend(old), tend(new)) and (tend(new), tend(new)):
+  // du rows, then dv rows, then dgd rows (types_mod.F90:347-371)
+  double ip1 = 0, ip2 = 0;
+  const int per_comp = M * a.nchunk_row;
+  for (int c = 0; c < 3; ++c) {
+    double q1 = 0, q2 = 0;
+    for (int k = tid; k < per_comp; k += kBlock) {
+      const int row = k / a.nchunk_row, ch = k % a.nchunk_row;
+      const double *pp = mb.part + ((size_t)(row * 3 + c) * a.nchunk_row + ch) * 2;
+      q1 += pp[0];
+      q2 += pp[1];
+    }
+    ip1 += block_sum(q1, red);
+    ip2 += block_sum(q2, red);
+  }
+  const double beta = (a.qcon && ip1 != 0.0 && ip2 != 0.0) ? ip1 / ip2 : 1.0;
+  const double dtb = a.dt * beta;
+  if (j == 0 && tid == 0) mb.beta[0] = beta;
+  const size_t o = (size_t)j * P;
+  const double *tgd = (const double *)mb.tc[CDGD];
+  const CT *tdu = (const CT *)mb.tc[CDU];
+  const CT *tdv = (const CT *)mb.tc[CDV];
+  for (int i = tid; i < N; i += kBlock) {
+    mb.st[a.s_out][FGD][o + i] = mb.st[a.s0][FGD][o + i] + dtb * tgd[o + i];
+    mb.st[a.s_out][FU][o + i] = mb.st[a.s0][FU][o + i] + dtb * (double)tdu[o + i];
+    if (j <= M - 2) mb.st[a.s_out][FV][o + i] = mb.st[a.s0][FV][o + i] + dtb * (double)tdv[o + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ordinary_diffusion (diffusion_mod.F90:74-217), order 2, FP64 in every mode
+// ---------------------------------------------------------------------------
+
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const int N = a.N, M = a.M, P = a.pitch;
+  const int m = a.m0 + t / M, j = t % M;
+  const Member &mb = a.mem[m];
+  extern __shared__ __align__(16) unsigned char smem[];
+  double *sG = (double *)smem;   // G rows j-1..j+2
+  double *su = sG + 4 * N;       // u rows j-1, j, j+1
+  double *sv = su + 3 * N;       // v half rows j-1, j, j+1
+  double *red = sv + 3 * N;      // 16
+  const int tid = threadIdx.x;
+  const double *gd = mb.st[a.s_in][FGD];
+  const double *Ug = mb.st[a.s_in][FU];
+  const double *Vg = mb.st[a.s_in][FV];
+  for (int r = 0; r < 4; ++r) {
+    const int row = j - 1 + r;
+    if (row < 0 || row > M - 1) continue;
+    for (int i = tid; i < N; i += kBlock) sG[r * N + i] = sqrt(gd[(size_t)row * P + i]);
+  }
+  __syncthreads();
+  const double *ug = mb.st[a.s_in][Fu], *vg = mb.st[a.s_in][Fv];
+  for (int r = 0; r < 3; ++r) {
+    const int row = j - 1 + r;
+    for (int i = tid; i < N; i += kBlock) {
+      double val = 0.0;
+      if (row >= 1 && row <= M - 2) {
+        const int ip = i + 1 == N ? 0 : i + 1;
+        val = a.in_full ? ug[(size_t)row * P + i]
+                        : Ug[(size_t)row * P + i] * 2.0 / (sG[r * N + i] + sG[r * N + ip]);
+      }
+      su[r * N + i] = val;
+    }
+    const int hrow = row;
+    for (int i = tid; i < N; i += kBlock) {
+      double val = 0.0;
+      if (hrow >= 0 && hrow <= M - 2)
+        val = a.in_full ? vg[(size_t)hrow * P + i]
+                        : Vg[(size_t)hrow * P + i] * 2.0 / (sG[r * N + i] + sG[(r + 1) * N + i]);
+      sv[r * N + i] = val;
+    }
+  }
+  __syncthreads();
+  const RowCoef cj = a.coef[j];
+  const bool has_u = j >= 1 && j <= M - 2;
+  const bool has_v = j <= M - 2;
+  const double hc_m = j >= 1 ? a.coef[j - 1].hcos : 0.0;
+  const double hc_0 = has_v ? cj.hcos : 0.0;
+  const double fc_p = j + 1 <= M - 1 ? a.coef[j + 1].fcos : 0.0;
+  // pole rows of gdd: zonal sums (diffusion_mod.F90:116-133)
+  double pole = 0.0;
+  if (j == 0 || j == M - 1) {
+    double acc = 0.0;
+    const int ja = j == 0 ? 1 : M - 1, jb = j == 0 ? 0 : M - 2;
+    for (int i = tid; i < N; i += kBlock) acc += gd[(size_t)ja * P + i] - gd[(size_t)jb * P + i];
+    double s = block_sum(acc, red);
+    if (j == 0)
+      pole = s * hc_0 / cj.fdlat2 * cj.fcos / N;
+    else
+      pole = -s * hc_m / cj.fdlat2 * cj.fcos / N;
+  }
+  const int st_u = has_u ? cj.stage[CDU] : -1;
+  const int st_v = has_v ? cj.stage[CDV] : -1;
+  const int st_g = cj.stage[CDGD];
+  CT *sx = (CT *)mb.sx;
+  const double kd = a.kdiff;
+  double *Xgd = mb.st[SX][FGD], *Xu = mb.st[SX][Fu], *Xv = mb.st[SX][Fv];
+  for (int i = tid; i < N; i += kBlock) {
+    const int im = i == 0 ? N - 1 : i - 1, ip = i == N - 1 ? 0 : i + 1;
+    const size_t o = (size_t)j * P;
+    double gdd;
+    if (j == 0 || j == M - 1) {
+      gdd = pole;
+    } else {
+      const double g0 = gd[o + i];
+      gdd = (gd[o + ip] - 2 * g0 + gd[o + im]) / cj.fdlon2 +
+            ((gd[o + P + i] - g0) * hc_0 - (g0 - gd[o - P + i]) * hc_m) / cj.fdlat2 * cj.fcos;
+    }
+    double ud = 0.0;
+    if (has_u) {
+      const double u0 = su[N + i];
+      ud = (su[N + ip] - 2 * u0 + su[N + im]) / cj.fdlon2 +
+           ((su[2 * N + i] - u0) * hc_0 - (u0 - su[i]) * hc_m) / cj.fdlat2 * cj.fcos;
+    }
+    double vd = 0.0;
+    if (has_v) {
+      const double v0 = sv[N + i];
+      vd = (sv[N + ip] - 2 * v0 + sv[N + im]) / cj.hdlon2;
+      if (j >= 1 && j <= M - 3)
+        vd = vd + ((sv[2 * N + i] - v0) * fc_p - (v0 - sv[i]) * cj.fcos) / cj.hdlat2 * cj.hcos;
+      else if (j == 0)
+        vd = vd + (sv[2 * N + i] - v0) * fc_p / cj.hdlat2 * cj.hcos;
+      else
+        vd = vd - (v0 - sv[i]) * cj.fcos / cj.hdlat2 * cj.hcos;
+    }
+    if (st_g >= 0)
+      sx[(size_t)st_g * P + i] = (CT)gdd;
+    else
+      Xgd[o + i] = gd[o + i] + kd * gdd;
+    if (st_u >= 0)
+      sx[(size_t)st_u * P + i] = (CT)ud;
+    else
+      Xu[o + i] = has_u ? su[N + i] + kd * ud : 0.0;
+    if (has_v) {
+      if (st_v >= 0)
+        sx[(size_t)st_v * P + i] = (CT)vd;
+      else
+        Xv[o + i] = sv[N + i] + kd * vd;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// end of step: iap_transform (types_mod.F90:399-426) or derivation of u, v,
+// then diag_run mass/energy (diag_mod.F90:42-121)
+// ---------------------------------------------------------------------------
+
+template <int MODE>  // 0: iap_transform after diffusion, 1: derive u,v from an IAP slot, 2: diag only
+__global__ void __launch_bounds__(kBlock) k_finish(const Args a) {
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const int N = a.N, M = a.M, P = a.pitch;
+  const int m = a.m0 + t / M, j = t % M;
+  const Member &mb = a.mem[m];
+  __shared__ double red[16];
+  __shared__ int last;
+  const int tid = threadIdx.x;
+  const RowCoef cj = a.coef[j];
+  const size_t o = (size_t)j * P;
+  double *Xgd = mb.st[SX][FGD], *XU = mb.st[SX][FU], *XV = mb.st[SX][FV];
+  double *Xu = mb.st[SX][Fu], *Xv = mb.st[SX][Fv];
+  const double *sgd = MODE != 1 ? Xgd : mb.st[a.s_in][FGD];
+  const double *sU = mb.st[a.s_in][FU], *sV = mb.st[a.s_in][FV];
+  double em = 0, eu = 0, ev = 0, eg = 0;
+  for (int i = tid; i < N; i += kBlock) {
+    const int ip = i == N - 1 ? 0 : i + 1;
+    const double g0 = sgd[o + i];
+    const double G0 = sqrt(g0), G1 = sqrt(sgd[o + ip]);
+    const double Gn = j <= M - 2 ? sqrt(sgd[o + P + i]) : 0.0;
+    double U, V = 0.0;
+    if (MODE == 2) {
+      U = XU[o + i];
+      if (j <= M - 2) V = XV[o + i];
+    } else if (MODE == 0) {
+      U = 0.5 * (G0 + G1) * Xu[o + i];
+      XU[o + i] = U;
+      if (j <= M - 2) {
+        V = 0.5 * (G0 + Gn) * Xv[o + i];
+        XV[o + i] = V;
+      }
+    } else {
+      U = sU[o + i];
+      Xgd[o + i] = g0;
+      XU[o + i] = U;
+      Xu[o + i] = (j == 0 || j == M - 1) ? 0.0 : U * 2.0 / (G0 + G1);
+      if (j <= M - 2) {
+        V = sV[o + i];
+        XV[o + i] = V;
+        Xv[o + i] = V * 2.0 / (G0 + Gn);
+      }
+    }
+    em += cj.mass_w * g0;
+    if (j >= 1 && j <= M - 2) eu += U * U * cj.fcos;
+    if (j <= M - 2) ev += V * V * cj.hcos;
+    const double h = g0 + mb.ghs[o + i];
+    eg += h * h * cj.fcos;
+  }
+  em = block_sum(em, red);
+  eu = block_sum(eu, red);
+  ev = block_sum(ev, red);
+  eg = block_sum(eg, red);
+  if (tid == 0) {
+    double *dp = mb.dpart + (size_t)j * 4;
+    dp[0] = em; dp[1] = eu; dp[2] = ev; dp[3] = eg;
+    __threadfence();
+    const uint32_t prev = atomicAdd(&a.done[m], 1u);
+    last = prev == (uint32_t)(M - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double q[4] = {0, 0, 0, 0};
+  for (int r = tid; r < M; r += kBlock)
+    for (int k = 0; k < 4; ++k) q[k] += __builtin_nontemporal_load(mb.dpart + (size_t)r * 4 + k);
+  for (int k = 0; k < 4; ++k) q[k] = block_sum(q[k], red);
+  if (tid == 0) {
+    const int s = a.step_ctr[m];
+    double *out = mb.diag_out + (size_t)(s % a.diag_cap) * 3;
+    out[0] = q[0] * (kRadius * kRadius);
+    out[1] = q[1] + q[2] + q[3];
+    out[2] = mb.beta[0];
+    a.step_ctr[m] = s + 1;
+    a.done[m] = 0;
+  }
+}
+
+// diag_run div and vor of state X (diag_mod.F90:49-77), history only
+__global__ void __launch_bounds__(kBlock) k_divvor(const Args a) {
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const int N = a.N, M = a.M, P = a.pitch;
+  const int m = a.m0 + t / M, j = t % M;
+  const Member &mb = a.mem[m];
+  const RowCoef cj = a.coef[j];
+  const double *u = mb.st[SX][Fu], *v = mb.st[SX][Fv];
+  const size_t o = (size_t)j * P;
+  for (int i = threadIdx.x; i < N; i += kBlock) {
+    const int im = i == 0 ? N - 1 : i - 1, ip = i == N - 1 ? 0 : i + 1;
+    if (j >= 1 && j <= M - 2) {
+      const double um1 = u[o + im], up1 = u[o + i];
+      const double vm1 = v[o - P + i] * a.coef[j - 1].hcos, vp1 = v[o + i] * cj.hcos;
+      mb.div[o + i] = (up1 - um1) / cj.fdlon + (vp1 - vm1) / cj.fdlat;
+    } else {
+      mb.div[o + i] = 0.0;
+    }
+    if (j <= M - 2) {
+      const double um1 = u[o + i], up1 = u[o + P + i];
+      const double vm1 = v[o + i], vp1 = v[o + ip];
+      mb.vor[o + i] = (vp1 - vm1) / cj.hdlon - (up1 - um1) / cj.hdlat;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// polar filter on MFMA
+// ---------------------------------------------------------------------------
+
+template <typename CT> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4_t acc_t;
+  static __device__ __forceinline__ acc_t run(double x, double y, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  typedef f4_t acc_t;
+  static __device__ __forceinline__ acc_t run(float x, float y, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+  }
+};
+
+// Accumulator layout of the 16x16x4 MFMAs: lane l (q = l/16) register r holds
+// D[acc_row(q, r)][l % 16].  The FP64 form interleaves rows (q + 4r), the
+// FP32 form blocks them (4q + r); checked on the device by k_mfma_layout.
+template <typename CT> __device__ __forceinline__ int acc_row(int q, int r);
+template <> __device__ __forceinline__ int acc_row<double>(int q, int r) { return q + 4 * r; }
+template <> __device__ __forceinline__ int acc_row<float>(int q, int r) { return 4 * q + r; }
+
+// FFTPACK5 real basis of slot s at longitude index l (0 outside [0,N)).
+template <typename CT>
+__device__ __forceinline__ CT basis(const CT *ctab, const CT *stab, int N, int s, int l) {
+  if (s >= N || l >= N) return (CT)0;
+  if (s == 0) return (CT)1;
+  const int k = (s + 1) >> 1;
+  const int q = (k * l) % N;
+  return (s & 1) ? ctab[q] : stab[q];
+}
+
+__device__ __forceinline__ double slot_weight(int N, int s) {
+  return (s == 0 || ((N & 1) == 0 && s == N - 1)) ? 1.0 / N : 2.0 / N;
+}
+
+// weight row of a staged row (the inner product partner of the rescale)
+__device__ __forceinline__ double weight_val(const Args &a, const Member &mb, int comp, int row, int l) {
+  const size_t o = (size_t)row * a.pitch + l;
+  if (comp == CDU) return mb.st[a.s_in][FU][o];
+  if (comp == CDV) return mb.st[a.s_in][FV][o];
+  return mb.st[a.s_in][FGD][o] + mb.ghs[o];
+}
+
+// F1: coefficients C = X B^T over one tile x one chunk of 64 slots, masked
+// and weighted; also the s2 partial sum_s w_s C_x[s] C_w[s] of each row.
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) k_filter_coef(const Args a) {
+  const int wi = blockIdx.x;
+  if (wi >= a.ntiles) return;
+  const int tile = wi / a.nchunk1, chunk = wi % a.nchunk1;
+  const FiltTile ft = a.tiles[tile];
+  const int N = a.N, P = a.pitch;
+  const int slo = ft.lo + chunk * kChunk;
+  if (slo >= ft.hi) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  CT *ctab = (CT *)smem;
+  CT *stab = ctab + N;
+  constexpr int LB = 64;
+  CT *xs = stab + N;  // [16][LB]
+  __shared__ double s2w[4][kFiltRows];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int q = tid; q < N; q += kBlock) {
+    ctab[q] = (CT)a.ftab[q];
+    stab[q] = (CT)a.ftab[N + q];
+  }
+  const bool want_w = a.phase <= PH_T;
+  typename Mfma<CT>::acc_t acc = {0, 0, 0, 0};
+  const int s_mine = slo + wave * 16 + (lane & 15);
+  const bool s_ok = s_mine < ft.hi;
+  for (int l0 = 0; l0 < N; l0 += LB) {
+    __syncthreads();
+    for (int e = tid; e < 16 * LB; e += kBlock) {
+      const int r = e / LB, l = l0 + e % LB;
+      CT v = (CT)0;
+      if (l < N) {
+        const int rr = r & 7;
+        if (rr < ft.nrows) {
+          const Member &mb = a.mem[ft.member[rr]];
+          const StageSlot ss = a.slots[ft.slot[rr]];
+          if (r < 8)
+            v = ((const CT *)mb.sx)[(size_t)ft.slot[rr] * P + l];
+          else if (want_w)
+            v = (CT)weight_val(a, mb, ss.comp, ss.row, l);
+        }
+      }
+      xs[e] = v;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < LB; kk += 4) {
+      const int k = kk + (lane >> 4);
+      const CT av = xs[(lane & 15) * LB + k];
+      const CT bv = s_ok ? basis<CT>(ctab, stab, N, s_mine, l0 + k) : (CT)0;
+      acc = Mfma<CT>::run(av, bv, acc);
+    }
+  }
+  // acc[r] = C[acc_row(q, r)][slot s_mine]; weight row = staged row + 8
+  const int q = lane >> 4;
+  double part[4];
+  double *fc = a.fcoef + (size_t)tile * kFiltRows * a.kpad;
+  const double ws = s_ok ? slot_weight(N, s_mine) : 0.0;
+  double accd[4], accw[4];
+  for (int r = 0; r < 4; ++r) accd[r] = (double)acc[r];
+  for (int r = 0; r < 4; ++r) {
+    if constexpr (sizeof(CT) == 8)
+      accw[r] = accd[(r + 2) & 3];            // FP64: row + 8 is register r + 2
+    else
+      accw[r] = __shfl_xor(accd[r], 32, 64);  // FP32: row + 8 is 32 lanes away
+  }
+  for (int r = 0; r < 4; ++r) {
+    const int row = acc_row<CT>(q, r);  // 0..15
+    const double cx = accd[r];
+    const double cw = accw[r];
+    part[r] = 0.0;
+    if (row < 8 && row < ft.nrows && s_ok) {
+      const StageSlot ss = a.slots[ft.slot[row]];
+      const bool keep = ft.complement ? s_mine >= ss.kept : s_mine < ss.kept;
+      const double cval = keep ? ws * cx : 0.0;
+      fc[(size_t)row * a.kpad + (s_mine - ft.lo)] = cval;
+      part[r] = cval * cw;
+    } else if (row < 8 && s_mine - ft.lo < a.kpad && s_mine < ft.lo + a.kpad) {
+      fc[(size_t)row * a.kpad + (s_mine - ft.lo)] = 0.0;
+    }
+  }
+  for (int r = 0; r < 4; ++r) part[r] = group16_sum(part[r]);
+  if ((lane & 15) == 0)
+    for (int r = 0; r < 4; ++r) {
+      const int row = acc_row<CT>(q, r);
+      if (row < kFiltRows) s2w[wave][row] = part[r];
+    }
+  __syncthreads();
+  if (tid < kFiltRows) {
+    const double s = ((s2w[0][tid] + s2w[1][tid]) + s2w[2][tid]) + s2w[3][tid];
+    a.fs2[((size_t)tile * a.nchunk1 + chunk) * kFiltRows + tid] = s;
+  }
+}
+
+// F2: reconstruction Y = C' B over one tile x 64 columns, the rescale
+// tend*s1/s2 (dycore_mod.F90:213-217) and the phase epilogue.
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) k_filter_apply(const Args a) {
+  const int wi = blockIdx.x;
+  if (wi >= a.ntiles) return;
+  const int tile = wi / a.nchunk2, chunk = wi % a.nchunk2;
+  const FiltTile ft = a.tiles[tile];
+  const int N = a.N, P = a.pitch;
+  extern __shared__ __align__(16) unsigned char smem[];
+  CT *ctab = (CT *)smem;
+  CT *stab = ctab + N;
+  CT *cs = stab + N;  // [8][kpad]
+  __shared__ double red[4][2][kFiltRows];
+  __shared__ double s2row[kFiltRows];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ks = ft.hi - ft.lo;
+  for (int q = tid; q < N; q += kBlock) {
+    ctab[q] = (CT)a.ftab[q];
+    stab[q] = (CT)a.ftab[N + q];
+  }
+  const double *fc = a.fcoef + (size_t)tile * kFiltRows * a.kpad;
+  for (int e = tid; e < kFiltRows * a.kpad; e += kBlock) cs[e] = (CT)fc[e];
+  if (tid < kFiltRows) {
+    double s = 0.0;
+    const int nch = (ks + kChunk - 1) / kChunk;
+    for (int c = 0; c < nch; ++c) s += a.fs2[((size_t)tile * a.nchunk1 + c) * kFiltRows + tid];
+    s2row[tid] = s;
+  }
+  __syncthreads();
+  const int col = chunk * kChunk + wave * 16 + (lane & 15);
+  typename Mfma<CT>::acc_t acc = {0, 0, 0, 0};
+  const int kp4 = (ks + 3) & ~3;
+  for (int s0 = 0; s0 < kp4; s0 += 4) {
+    const int k = s0 + (lane >> 4);
+    const int row = lane & 15;
+    const CT av = (row < 8 && k < ks) ? cs[row * a.kpad + k] : (CT)0;
+    const CT bv = (k < ks) ? basis<CT>(ctab, stab, N, ft.lo + k, col) : (CT)0;
+    acc = Mfma<CT>::run(av, bv, acc);
+  }
+  const int q = lane >> 4;
+  double p1[4] = {0, 0, 0, 0}, p2[4] = {0, 0, 0, 0};
+  if (col < N) {
+    for (int r = 0; r < 4; ++r) {
+      const int row = acc_row<CT>(q, r);
+      if (row >= ft.nrows) continue;
+      const Member &mb = a.mem[ft.member[row]];
+      const int slot = ft.slot[row];
+      const StageSlot ss = a.slots[slot];
+      const double x = (double)((const CT *)mb.sx)[(size_t)slot * P + col];
+      const double y = ft.complement ? x - (double)acc[r] : (double)acc[r];
+      double T;
+      if (a.phase <= PH_T) {
+        const double s1 = mb.s1[slot];
+        const double s2 = ft.complement ? s1 - s2row[row] : s2row[row];
+        T = fabs(s1) > 1.0e-16 ? y * s1 / s2 : x;
+      } else {
+        T = y;
+      }
+      const size_t o = (size_t)ss.row * P + col;
+      if (a.phase <= PH_T) {
+        const double w = ss.comp == CDV ? a.coef[ss.row].hcos : a.coef[ss.row].fcos;
+        phase_epilogue<CT>(a, mb, ss.comp, ss.row, col, T, w, p1[r], p2[r]);
+      } else if (a.phase == PH_D) {
+        // ordinary_diffusion update of the filtered component
+        const double *sgd = mb.st[a.s_in][FGD];
+        if (ss.comp == CDGD) {
+          mb.st[SX][FGD][o] = sgd[o] + a.kdiff * T;
+        } else if (ss.comp == CDU) {
+          const int ip = col + 1 == N ? 0 : col + 1;
+          const double u = a.in_full ? mb.st[a.s_in][Fu][o]
+                                     : mb.st[a.s_in][FU][o] * 2.0 / (sqrt(sgd[o]) + sqrt(sgd[o - col + ip]));
+          mb.st[SX][Fu][o] = u + a.kdiff * T;
+        } else {
+          const double v = a.in_full ? mb.st[a.s_in][Fv][o]
+                                     : mb.st[a.s_in][FV][o] * 2.0 / (sqrt(sgd[o]) + sqrt(sgd[o + P]));
+          mb.st[SX][Fv][o] = v + a.kdiff * T;
+        }
+      } else {  // PH_P: projection only, result back into the staging row
+        ((CT *)mb.sx)[(size_t)slot * P + col] = (CT)T;
+      }
+    }
+  }
+  if (a.phase == PH_C) {
+    for (int r = 0; r < 4; ++r) {
+      p1[r] = group16_sum(p1[r]);
+      p2[r] = group16_sum(p2[r]);
+    }
+    if ((lane & 15) == 0)
+      for (int r = 0; r < 4; ++r) {
+        const int row = acc_row<CT>(q, r);
+        if (row < kFiltRows) {
+          red[wave][0][row] = p1[r];
+          red[wave][1][row] = p2[r];
+        }
+      }
+    __syncthreads();
+    if (tid < ft.nrows) {
+      const Member &mb = a.mem[ft.member[tid]];
+      const StageSlot ss = a.slots[ft.slot[tid]];
+      double *pp = mb.part + ((size_t)(ss.row * 3 + ss.comp) * a.nchunk_row + chunk) * 2;
+      pp[0] = ((red[0][0][tid] + red[1][0][tid]) + red[2][0][tid]) + red[3][0][tid];
+      pp[1] = ((red[0][1][tid] + red[1][1][tid]) + red[2][1][tid]) + red[3][1][tid];
+    }
+  }
+}
+
+// MFMA operand-layout check used by the tests: D = A B for 16x4 A, 4x16 B.
+__global__ void k_mfma_layout(const double *A, const double *B, double *D, const float *Af,
+                              const float *Bf, float *Df) {
+  const int lane = threadIdx.x;
+  d4_t acc = {0, 0, 0, 0};
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(lane & 15) * 4 + (lane >> 4)],
+                                             B[(lane >> 4) * 16 + (lane & 15)], acc, 0, 0, 0);
+  f4_t accf = {0, 0, 0, 0};
+  accf = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[(lane & 15) * 4 + (lane >> 4)],
+                                              Bf[(lane >> 4) * 16 + (lane & 15)], accf, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) {
+    D[acc_row<double>(lane >> 4, r) * 16 + (lane & 15)] = acc[r];
+    Df[acc_row<float>(lane >> 4, r) * 16 + (lane & 15)] = accf[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static inline int row_grid(int ntiles) { return ((ntiles + kXcd - 1) / kXcd) * kXcd; }
+
+template <typename CT>
+static hipError_t launch_tend_t(const Args &a, int pass, bool in_full, hipStream_t s) {
+  const size_t lds = tend_lds_bytes(a.N, sizeof(CT));
+  dim3 g(row_grid(a.ntiles)), b(kBlock);
+#define GD_TEND(P, F) hipLaunchKernelGGL((k_tend<CT, P, F>), g, b, lds, s, a)
+  if (pass == 0) { if (in_full) GD_TEND(0, true); else GD_TEND(0, false); }
+  else if (pass == 1) { if (in_full) GD_TEND(1, true); else GD_TEND(1, false); }
+  else { if (in_full) GD_TEND(2, true); else GD_TEND(2, false); }
+#undef GD_TEND
+  return hipGetLastError();
+}
+
+hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStream_t s) {
+  return prec == 0 ? launch_tend_t<double>(a, pass, in_full, s) : launch_tend_t<float>(a, pass, in_full, s);
+}
+
+hipError_t launch_correct(const Args &a, int prec, hipStream_t s) {
+  dim3 g(row_grid(a.ntiles)), b(kBlock);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_correct<double>, g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_correct<float>, g, b, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_diffuse(const Args &a, int prec, hipStream_t s) {
+  dim3 g(row_grid(a.ntiles)), b(kBlock);
+  const size_t lds = diffuse_lds_bytes(a.N);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_diffuse<double>, g, b, lds, s, a);
+  else
+    hipLaunchKernelGGL(k_diffuse<float>, g, b, lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_finish(const Args &a, int mode, hipStream_t s) {
+  dim3 g(row_grid(a.ntiles)), b(kBlock);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_finish<0>, g, b, 0, s, a);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_finish<1>, g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_finish<2>, g, b, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_divvor(const Args &a, hipStream_t s) {
+  dim3 g(row_grid(a.ntiles)), b(kBlock);
+  hipLaunchKernelGGL(k_divvor, g, b, 0, s, a);
+  return hipGetLastError();
+}
+
+// Raise the dynamic-LDS limit of every kernel that may need more than the
+// default (large num_lon).
+hipError_t configure_kernels(int max_lds) {
+  const void *fns[] = {
+      (const void *)k_tend<double, 0, true>, (const void *)k_tend<double, 0, false>,
+      (const void *)k_tend<double, 1, true>, (const void *)k_tend<double, 1, false>,
+      (const void *)k_tend<double, 2, true>, (const void *)k_tend<double, 2, false>,
+      (const void *)k_tend<float, 0, true>,  (const void *)k_tend<float, 0, false>,
+      (const void *)k_tend<float, 1, true>,  (const void *)k_tend<float, 1, false>,
+      (const void *)k_tend<float, 2, true>,  (const void *)k_tend<float, 2, false>,
+      (const void *)k_diffuse<double>,       (const void *)k_diffuse<float>,
+      (const void *)k_filter_coef<double>,   (const void *)k_filter_coef<float>,
+      (const void *)k_filter_apply<double>,  (const void *)k_filter_apply<float>};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_filter_coef(const Args &a, int prec, hipStream_t s) {
+  dim3 g(a.ntiles), b(kBlock);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_filter_coef<double>, g, b, filt_coef_lds_bytes(a.N, 8), s, a);
+  else
+    hipLaunchKernelGGL(k_filter_coef<float>, g, b, filt_coef_lds_bytes(a.N, 4), s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_apply(const Args &a, int prec, hipStream_t s) {
+  dim3 g(a.ntiles), b(kBlock);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_filter_apply<double>, g, b, filt_apply_lds_bytes(a.N, a.kpad, 8), s, a);
+  else
+    hipLaunchKernelGGL(k_filter_apply<float>, g, b, filt_apply_lds_bytes(a.N, a.kpad, 4), s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mfma_layout(const double *A, const double *B, double *D, const float *Af,
+                              const float *Bf, float *Df, hipStream_t s) {
+  hipLaunchKernelGGL(k_mfma_layout, dim3(1), dim3(64), 0, s, A, B, D, Af, Bf, Df);
+  return hipGetLastError();
+}
+
+}  // namespace gd

@@ -1,0 +1,209 @@
+/*
+ * gamil_dycore.h -- C ABI of the MI355X barotropic time step.
+ *
+ * Drop-in boundary for the time-stepping path of gamil-dycore.  The reference
+ * has no FFI: its seam is the argument-less Fortran facade
+ *   dycore_init / dycore_restart / dycore_run / dycore_final
+ * (src/dycore_mod.F90:22-25, 60-157) fed through module globals (the
+ * &dycore_params namelist, src/params_mod.F90:68-98; state(1) and static%ghs
+ * written by the test-case IC modules, src/data_mod.F90:19-22).  The
+ * functions below are what that facade binds through iso_c_binding
+ * (gamil-dycore_amd/fortran/gd_capi_mod.F90); every type is a plain C type.
+ *
+ * Conventions
+ *   - Return value: GD_OK (0) or a GD_ERR_* code; gd_last_error() gives text.
+ *   - The caller owns host buffers, the library owns device buffers; no host
+ *     pointer is kept after a call returns.
+ *   - Host field layout: pointer to element (i=1, j=1), longitude fastest,
+ *     consecutive latitude rows `ld` elements apart (ld >= num_lon).  A
+ *     Fortran halo'd array x(-1:N+2,-1:M+2) is passed as x(1,1) with
+ *     ld = N+4; a dense array as ld = N.  Full-lat fields (u, gd, ghs) have
+ *     num_lat rows, half-lat fields (v) num_lat-1 rows.
+ *   - A handle is bound to one HIP device and one stream and is not
+ *     thread-safe; several handles may coexist.
+ *   - A handle holds `n_members` independent runs of one configuration
+ *     (ensemble members); member 0 is the single-run case.
+ */
+#ifndef GAMIL_DYCORE_H
+#define GAMIL_DYCORE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GD_OK = 0,
+  GD_ERR_INVALID_ARG = 1,  /* bad config / size / pointer                    */
+  GD_ERR_HIP = 2,          /* HIP runtime error (device, memory, launch)      */
+  GD_ERR_NAN = 3,          /* total mass or energy became NaN (diag_mod.F90:79-87) */
+  GD_ERR_NOT_SUPPORTED = 4 /* option of the reference not provided here       */
+};
+
+enum { GD_SPLIT_NONE = 0, GD_SPLIT_CSP1 = 1, GD_SPLIT_CSP2 = 2, GD_SPLIT_ISP = 3 };
+enum { GD_ADV_CENTER_DIFF = 0, GD_ADV_UPWIND = 1, GD_ADV_WENO = 2 };
+enum { GD_PASS_ALL = 0, GD_PASS_FAST = 1, GD_PASS_SLOW = 2 };
+enum {
+  GD_PREC_FP64 = 0,        /* all arithmetic in FP64 (reference -fdefault-real-8) */
+  GD_PREC_MIXED = 1,       /* FP32 stencils/tendencies/filter; FP64 state update,
+                              pole sums, mass flux divergence and integrals      */
+  GD_PREC_MIXED_BF16 = 2   /* as MIXED, filter products in BF16 with FP32 accumulation */
+};
+
+/* &dycore_params (src/params_mod.F90:13-98) plus the device knobs. */
+typedef struct gd_config {
+  int32_t num_lon;                 /* num_lon                                   */
+  int32_t num_lat;                 /* num_lat (full latitudes incl. poles)       */
+  double time_step_size;           /* time_step_size [s]                         */
+  int32_t split_scheme;            /* split_scheme: GD_SPLIT_*                   */
+  int32_t subcycles;               /* subcycles                                  */
+  int32_t qcon_modified;           /* qcon_modified (logical)                    */
+  int32_t uv_adv_scheme;           /* uv_adv_scheme: GD_ADV_*                    */
+  double uv_adv_upwind_lon_beta;   /* uv_adv_upwind_lon_beta                     */
+  double uv_adv_upwind_lat_beta;   /* uv_adv_upwind_lat_beta                     */
+  int32_t use_diffusion;           /* use_diffusion (logical)                    */
+  int32_t diffusion_order;         /* diffusion_order                            */
+  double diffusion_coef;           /* diffusion_coef [m2/s]                      */
+  int32_t use_zonal_tend_filter;   /* use_zonal_tend_filter (logical)            */
+  int32_t n_cutoff;                /* number of entries in cutoff[]              */
+  const int32_t *cutoff;           /* zonal_tend_filter_cutoff_wavenumber(1:n),
+                                      counted from each pole                      */
+  int32_t precision;               /* GD_PREC_*                                  */
+  int32_t device;                  /* HIP device ordinal                         */
+} gd_config;
+
+typedef struct gd_handle gd_handle;
+
+/* dycore_init (dycore_mod.F90:60-111): mesh_init, data_init, filter_init and
+ * the integrator/split/advection selection, for n_members runs.  Also applies
+ * reset_cos_lat_at_poles (dycore_mod.F90:159-173), which dycore_run performs
+ * before its first use of the coefficients. */
+int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out);
+
+/* dycore_final (dycore_mod.F90:144-157). */
+int gd_destroy(gd_handle *h);
+
+/* Text of the last error on this handle (or of the last failed gd_create on
+ * this thread when h is NULL). */
+const char *gd_last_error(const gd_handle *h);
+
+/* IC hand-over: the writes of the test-case modules into state(1)%{u,v,gd}
+ * and static%ghs (e.g. rossby_haurwitz_wave_test_mod.F90:33-85), followed by
+ * iap_transform(state(old)) (dycore_mod.F90:123, types_mod.F90:399-426) on
+ * the device.  u must vanish on the two pole rows (the reference never
+ * updates u there, dycore_mod.F90:207,636). */
+int gd_set_state(gd_handle *h, int32_t member, const double *u, const double *v,
+                 const double *gd, const double *ghs, int64_t ld);
+
+/* diag_run (diag_mod.F90:42-89) of the current state: total mass and energy
+ * for every member (arrays of n_members). */
+int gd_diag(gd_handle *h, double *total_mass, double *total_energy);
+
+/* The body of the dycore_run loop (dycore_mod.F90:131-140) for nsteps steps:
+ * time_integrate (654-669: split scheme + predict_correct + diffusion),
+ * time_advance and diag_run.  Outputs (any may be NULL) are laid out
+ * [step][member]: total mass, total energy and the beta of the last
+ * integrator call of the step (the value log_step prints).  Returns
+ * GD_ERR_NAN if mass or energy became NaN; *nan_step (may be NULL) then holds
+ * the 1-based first such step. */
+int gd_step(gd_handle *h, int32_t nsteps, double *total_mass, double *total_energy,
+            double *beta, int32_t *nan_step);
+
+/* D2H copy of the prognostic state (u, v, gd; any pointer may be NULL) and
+ * of the history diagnostics div and vor (diag_mod.F90:49-77, may be NULL). */
+int gd_get_state(gd_handle *h, int32_t member, double *u, double *v, double *gd,
+                 int64_t ld);
+int gd_get_diag_fields(gd_handle *h, int32_t member, double *div, double *vor,
+                       int64_t ld);
+
+/* ---- operator seams (the reference's internal procedures) --------------- */
+
+/* space_operators(state(old), tend, pass) (dycore_mod.F90:184-365) including
+ * the polar filter and its inner-product rescale; du (num_lat rows, pole rows
+ * 0), dv (num_lat-1 rows), dgd (num_lat rows).  The state is not changed. */
+int gd_space_operators(gd_handle *h, int32_t member, int32_t pass, double *du,
+                       double *dv, double *dgd, int64_t ld);
+
+/* One integrator call predict_correct(dt, old, new, pass)
+ * (dycore_mod.F90:754-792) whose result becomes the current state; beta
+ * (n_members values, may be NULL) as logged at dycore_mod.F90:787. */
+int gd_predict_correct(gd_handle *h, double dt, int32_t pass, int32_t qcon_modified,
+                       double *beta);
+
+/* ordinary_diffusion(dt, state) (diffusion_mod.F90:74-217) applied to the
+ * current state, including its closing iap_transform. */
+int gd_ordinary_diffusion(gd_handle *h, double dt);
+
+/* filter_array_at_full_lat / _half_lat (filter_mod.F90:105-167) applied to
+ * every flagged row of x (half = 0: num_lat rows; half = 1: num_lat-1 rows;
+ * unflagged rows are left as they are).  Runs the device filter kernel. */
+int gd_filter_rows(gd_handle *h, int32_t half, double *x, int64_t ld);
+
+/* Per-row filter table as set up by filter_init (filter_mod.F90:35-103):
+ * flag (0/1) and kept cutoff wavenumber c (-1 when the row has no mask) for
+ * full rows (num_lat) and half rows (num_lat-1). */
+int gd_filter_table(const gd_handle *h, int32_t *full_flag, int32_t *half_flag,
+                    int32_t *full_cut, int32_t *half_cut);
+
+/* ---- initial conditions (one-time host setup, not on the time step) ---- */
+
+/* Seeded geostrophically balanced state: height perturbation from random
+ * spherical-harmonic coefficients (total wavenumber 1..nmax, coefficient
+ * standard deviation ~ n^slope, tapered by sin^2(lat) so the geostrophic
+ * wind stays finite at the equator, rescaled to an area-weighted rms of
+ * rms_height m) on a mean depth of mean_depth m; winds in geostrophic balance
+ * with it.  n_mountains > 0 adds Gaussian mountains (height uniform in
+ * [500, 2000] m, half-width uniform in [10, 20] deg, centre latitude uniform
+ * in [-60, 60] deg) drawn from mountain_seed; ghs = g*hs and
+ * gd = g*(mean_depth + h') - ghs.  When pert_rms > 0 a second independent
+ * balanced field (seed pert_seed, spectrum to pert_nmax, rms pert_rms m) is
+ * added: ensemble members on a common state.  u is 0 on the pole rows. */
+typedef struct gd_ic_params {
+  uint64_t seed;
+  int32_t nmax;
+  double slope;
+  double rms_height;
+  double mean_depth;
+  int32_t n_mountains;
+  uint64_t mountain_seed;
+  uint64_t pert_seed;
+  int32_t pert_nmax;
+  double pert_rms;
+} gd_ic_params;
+
+int gd_ic_balanced(int32_t num_lon, int32_t num_lat, const gd_ic_params *p, double *u,
+                   double *v, double *gd, double *ghs, int64_t ld);
+
+/* The reference test cases with closed forms:
+ * rossby_haurwitz_wave_test_mod.F90:33-85 (R, omg, gd0) and
+ * mountain_zonal_flow_test_mod.F90:28-97 (smooth_mountain = .false.). */
+int gd_ic_rossby_haurwitz(int32_t num_lon, int32_t num_lat, double R, double omg,
+                          double gd0, double *u, double *v, double *gd, double *ghs,
+                          int64_t ld);
+int gd_ic_mountain_zonal_flow(int32_t num_lon, int32_t num_lat, double *u, double *v,
+                              double *gd, double *ghs, int64_t ld);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* Runs nsteps steps launching every kernel eagerly with HIP events around
+ * each launch (same stream) and returns, per kernel family, the number of
+ * launches and the summed milliseconds.  names receives a comma-separated
+ * list of the family names (capacity name_cap bytes). */
+int gd_profile_kernels(gd_handle *h, int32_t nsteps, int32_t max_families,
+                       int32_t *n_families, int64_t *launches, double *ms,
+                       char *names, int32_t name_cap);
+
+/* Device bytes held by the handle (all members). */
+int64_t gd_device_bytes(const gd_handle *h);
+
+/* Device self-test of the MFMA operand layout the filter kernels use:
+ * D = A B for A 16x4 and B 4x16 (row-major) through one 16x16x4 MFMA in
+ * FP64 (A, B, D) and FP32 (Af, Bf, Df) on the current device. */
+int gd_selftest_mfma(const double *A, const double *B, double *D, const float *Af,
+                     const float *Bf, float *Df);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,140 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+FP64 mode: the device evaluates the reference expressions with the same
+operation order and no FMA contraction, so pointwise results agree to the
+last bits; only sums (pole fluxes, filter products, the rescale and beta
+inner products) run in another order.  Tolerances below are therefore
+close to FP64 round-off: 1e-13 relative for one operator evaluation and
+1e-12 .. 1e-11 for short integrations.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(gd, oracle_mod, N, M, dt, ic="balanced", seed=3, nmax=10, mountains=0, **kw):
+    if ic == "balanced":
+        u, v, h, ghs = gd.ic_balanced(N, M, seed=seed, nmax=nmax, n_mountains=mountains,
+                                      mountain_seed=seed + 100)
+    elif ic == "rh":
+        u, v, h, ghs = gd.ic_rossby_haurwitz(N, M)
+    else:
+        u, v, h, ghs = gd.ic_mountain_zonal_flow(N, M)
+    cut = kw.pop("cutoff", [])
+    o = oracle_mod.Oracle(N, M, dt, cutoff=cut, **kw)
+    o.set_initial_state(u, v, h, ghs)
+    o.start()
+    d = gd.Dycore(N, M, dt, zonal_tend_filter_cutoff_wavenumber=cut, **kw)
+    d.set_state(u, v, h, ghs)
+    return o, d
+
+
+def test_mfma_operand_layout(gd):
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((16, 4)); B = rng.standard_normal((4, 16))
+    D, Df = gd.selftest_mfma(A, B, A.astype(np.float32), B.astype(np.float32))
+    assert np.abs(D - A @ B).max() < 1e-13
+    assert np.abs(Df - (A.astype(np.float32) @ B.astype(np.float32))).max() < 1e-5
+
+
+@pytest.mark.parametrize("pass_,pid", [("all", 0), ("fast", 1), ("slow", 2)])
+@pytest.mark.parametrize("adv", ["center_diff", "upwind"])
+def test_space_operators(gd, oracle_mod, pass_, pid, adv):
+    N, M = 72, 37
+    o, d = _pair(gd, oracle_mod, N, M, 600.0, uv_adv_scheme=adv, mountains=2, cutoff=[5, 4, 3, 2])
+    o.space_operators(o.old, 0, pid)
+    du, dv, dgd = d.space_operators(pass_)
+    T = oracle_mod
+    for name, a, b in (("du", du, o.tend(0, T.T_DU)), ("dv", dv, o.tend(0, T.T_DV)),
+                       ("dgd", dgd, o.tend(0, T.T_DGD))):
+        err = np.abs(a - b).max(axis=1) / max(np.abs(b).max(), 1e-300)
+        print(name, "worst rows", np.argsort(err)[-3:], err[np.argsort(err)[-3:]])
+        assert err.max() < 1e-12, name
+    if pass_ == "slow":
+        assert np.all(dgd == 0)
+    d.close()
+
+
+def test_space_operators_unfiltered_rows_bitwise(gd, oracle_mod):
+    """Rows off the filter and off the poles involve no reduction: identical bits."""
+    N, M = 96, 49
+    o, d = _pair(gd, oracle_mod, N, M, 600.0, cutoff=[4, 3])
+    o.space_operators(o.old, 0, 0)
+    du, dv, dgd = d.space_operators("all")
+    ff, hf, _, _ = d.filter_table()
+    rows = [j for j in range(2, M - 2) if not ff[j] and not ff[j - 1] and not ff[j + 1]]
+    T = oracle_mod
+    np.testing.assert_array_equal(du[rows], o.tend(0, T.T_DU)[rows])
+    np.testing.assert_array_equal(dgd[rows], o.tend(0, T.T_DGD)[rows])
+    hrows = [j for j in rows if not hf[j]]
+    np.testing.assert_array_equal(dv[hrows], o.tend(0, T.T_DV)[hrows])
+    d.close()
+
+
+@pytest.mark.parametrize("N,M,cut", [(72, 37, [6, 5, 4, 3, 2, 1]), (128, 65, [30, 20, 10, 60, 61]),
+                                     (90, 46, [44, 1, 43])])
+def test_filter_rows(gd, oracle_mod, N, M, cut):
+    o, d = _pair(gd, oracle_mod, N, M, 600.0, cutoff=cut)
+    rng = np.random.default_rng(1)
+    ff, hf, _, _ = d.filter_table()
+    x = rng.standard_normal((M, N))
+    y = d.filter_rows(x, half=False)
+    for j in range(M):
+        ref = o.filter_full(j + 1, x[j]) if ff[j] else x[j]
+        assert rel_err(y[j], ref) < 1e-13, j
+    xh = rng.standard_normal((M - 1, N))
+    yh = d.filter_rows(xh, half=True)
+    for j in range(M - 1):
+        ref = o.filter_half(j + 1, xh[j]) if hf[j] else xh[j]
+        assert rel_err(yh[j], ref) < 1e-13, j
+    d.close()
+
+
+@pytest.mark.parametrize("pass_,pid,dt", [("all", 0, 300.0), ("fast", 1, 100.0), ("slow", 2, 300.0)])
+def test_predict_correct(gd, oracle_mod, pass_, pid, dt):
+    N, M = 72, 37
+    o, d = _pair(gd, oracle_mod, N, M, 600.0, mountains=1, cutoff=[6, 4, 2])
+    beta_ref = o.predict_correct(dt, o.old, o.new, pid, True)
+    beta = d.predict_correct(dt, pass_)
+    assert abs(beta[0] - beta_ref) < 1e-13 * abs(beta_ref)
+    u, v, h = d.get_state()
+    assert rel_err(h, o.state(o.new, oracle_mod.GD)) < 1e-14
+    assert rel_err(u, o.state(o.new, oracle_mod.U)) < 1e-12
+    assert rel_err(v, o.state(o.new, oracle_mod.V)) < 1e-12
+    d.close()
+
+
+def test_ordinary_diffusion(gd, oracle_mod):
+    N, M = 72, 37
+    o, d = _pair(gd, oracle_mod, N, M, 600.0, use_diffusion=True, diffusion_coef=5e5,
+                 cutoff=[5, 3, 2])
+    o.ordinary_diffusion(600.0, o.old)
+    d.ordinary_diffusion(600.0)
+    u, v, h = d.get_state()
+    assert rel_err(h, o.state(o.old, oracle_mod.GD)) < 1e-14
+    assert rel_err(u, o.state(o.old, oracle_mod.U)) < 1e-12
+    assert rel_err(v, o.state(o.old, oracle_mod.V)) < 1e-12
+    d.close()
+
+
+@pytest.mark.parametrize("split", ["csp2", "none"])
+def test_steps_vs_oracle(gd, oracle_mod, split):
+    N, M, dt = 72, 37, 900.0
+    kw = dict(split_scheme=split, subcycles=3, use_diffusion=True, diffusion_coef=2e5,
+              cutoff=[6, 5, 4, 3, 2])
+    o, d = _pair(gd, oracle_mod, N, M, dt, mountains=2, **kw)
+    om, oe, ob, ns = o.run(8)
+    mass, energy, beta, _ = d.step(8)
+    u, v, h = d.get_state()
+    # 8 steps x 5 integrator calls: the ~1e-13 reduction-order differences of
+    # the filtered rows compound through the dynamics
+    assert rel_err(h, o.state(o.old, oracle_mod.GD)) < 1e-11
+    assert rel_err(u, o.state(o.old, oracle_mod.U)) < 1e-10
+    assert rel_err(v, o.state(o.old, oracle_mod.V)) < 1e-10
+    assert np.all(np.abs(mass[:, 0] - om) <= 1e-13 * np.abs(om))
+    assert np.all(np.abs(energy[:, 0] - oe) <= 1e-12 * np.abs(oe))
+    assert np.all(np.abs(beta[:, 0] - ob) <= 1e-10)
+    d.close()
