@@ -1,0 +1,194 @@
+"""Benchmark: model-days/sec of the barotropic step at 720x361 (0.5 deg),
+dt = 120 s, 10 model days (7200 steps), csp2 with 6 subcycles, polar filter
+and diffusion on; mixed precision (the headline value) against plain FP64 on
+the same GPU, plus p50/p99 run latency over repeated full runs.
+
+Prints ONE JSON line (rank 0).  A "step" is one time step of the path over
+the whole grid (time_integrate + diag_run, dycore_mod.F90:131-140).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("model-days/sec at 720x361 (mixed vs FP64 plain, same GPU); "
+          "p50/p99 run latency")
+
+# The 720x361 workload; values not fixed by the workload
+# description are stated here and in namelists/cfg2_720x361.nml.
+WORKLOAD = dict(num_lon=720, num_lat=361, time_step_size=120.0, split_scheme="csp2",
+                subcycles=6, qcon_modified=True, uv_adv_scheme="center_diff",
+                use_diffusion=True, diffusion_order=2, diffusion_coef=2.5e4,
+                use_zonal_tend_filter=True)
+FILTER_LAT0 = 75.0
+IC = dict(seed=20241, nmax=20, slope=-3.0, rms_height=150.0, mean_depth=8000.0,
+          n_mountains=3, mountain_seed=777)
+MODEL_DAYS = 10
+
+
+def algorithmic_bytes_per_step(N, M, S, prec):
+    """Algorithmic HBM bytes of the tendency kernel family per step.
+
+    One predict_correct call runs the tendency kernel three times:
+      A: read S0 (gd, U, V) + ghs, write S1 (gd, U, V)                 7 field passes
+      B: read S1 + ghs + S0, write S2 + Tb (du, dv, dgd)               13
+      C: read S2 + ghs + Tb, write Tc                                  10
+    The slow pass has no pressure gradient and skips ghs (6, 12, 9).
+    A csp2 step has 2 slow and S fast calls.  State fields are FP64; in mixed
+    mode du/dv tendencies are FP32 (dgd stays FP64).
+    """
+    full = N * M            # u, gd, U points
+    half = N * (M - 1)      # v, V points
+    st = 8 * (full + full + half)              # gd, U, V
+    ghs = 8 * full
+    tsz = 8 if prec == "fp64" else 4
+    tend = tsz * (full + half) + 8 * full      # du, dv, dgd
+
+    def call(slow):
+        g = 0 if slow else ghs
+        a = st + g + st
+        b = st + g + st + st + tend
+        c = st + g + tend + tend
+        return a + b + c
+
+    return 2 * call(True) + S * call(False)
+
+
+def make_dycore(gd, prec, members=1):
+    cut = gd.cutoff_from_lat0(WORKLOAD["num_lon"], WORKLOAD["num_lat"], FILTER_LAT0)
+    return gd.Dycore(zonal_tend_filter_cutoff_wavenumber=cut, precision=prec, members=members,
+                     **WORKLOAD), cut
+
+
+def run_gpu(gd, prec, ic, steps, warmup, repeats):
+    d, cut = make_dycore(gd, prec)
+    u, v, h, ghs = ic
+    d.set_state(u, v, h, ghs)
+    if warmup:
+        d.step(warmup)
+    d.set_state(u, v, h, ghs)
+    t0 = time.perf_counter()
+    mass, energy, beta, _ = d.step(steps)   # returns after a stream synchronize
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    full = MODEL_DAYS * 86400 / WORKLOAD["time_step_size"]
+    lat = []
+    for _ in range(repeats):
+        d.set_state(u, v, h, ghs)
+        s0 = time.perf_counter()
+        d.step(int(full))
+        lat.append(time.perf_counter() - s0)
+    prof = d.profile_kernels(60)
+    d.close()
+    return dict(elapsed=elapsed, mass=mass[:, 0], energy=energy[:, 0], lat=lat, prof=prof, cut=cut)
+
+
+def cpu_baseline(ic, nsteps):
+    """The FP64 C oracle (single thread, the reference is serial) on a bounded
+    sample of the same workload: the first nsteps steps."""
+    from oracle.oracle import Oracle, set_threads
+    set_threads(1)
+    cut = None
+    import importlib
+    gd = importlib.import_module("gamil_dycore_amd")
+    cut = gd.cutoff_from_lat0(WORKLOAD["num_lon"], WORKLOAD["num_lat"], FILTER_LAT0)
+    kw = {k: v for k, v in WORKLOAD.items() if k not in ("num_lon", "num_lat", "time_step_size")}
+    o = Oracle(WORKLOAD["num_lon"], WORKLOAD["num_lat"], WORKLOAD["time_step_size"],
+               split_scheme=kw["split_scheme"], subcycles=kw["subcycles"],
+               qcon_modified=kw["qcon_modified"], uv_adv_scheme=kw["uv_adv_scheme"],
+               use_diffusion=kw["use_diffusion"], diffusion_order=kw["diffusion_order"],
+               diffusion_coef=kw["diffusion_coef"], use_zonal_tend_filter=True, cutoff=cut)
+    u, v, h, ghs = ic
+    o.set_initial_state(u, v, h, ghs)
+    o.start()
+    t0 = time.perf_counter()
+    o.run(nsteps)
+    el = time.perf_counter() - t0
+    return nsteps * WORKLOAD["time_step_size"] / 86400.0 / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=7200)
+    ap.add_argument("--warmup", type=int, default=360)
+    ap.add_argument("--repeats", type=int, default=3, help="full 10-day runs for p50/p99 latency")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    import __graft_entry__ as ge
+    gd = ge.load_package()
+    N, M, dt = WORKLOAD["num_lon"], WORKLOAD["num_lat"], WORKLOAD["time_step_size"]
+    ic = gd.ic_balanced(N, M, **IC)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        res[prec] = run_gpu(gd, prec, ic, args.steps, args.warmup, args.repeats)
+    days = args.steps * dt / 86400.0
+    val = {p: days / r["elapsed"] for p, r in res.items()}
+    mixed = res["mixed"]
+    # roofline of the dominant kernel family (tendency kernel), from event-timed launches
+    prof = mixed["prof"]
+    dom = max(prof, key=lambda k: prof[k][1])
+    n_launch, ms = prof["tend"]
+    per_step_launch = n_launch / 60.0
+    bytes_step = algorithmic_bytes_per_step(N, M, WORKLOAD["subcycles"], "mixed")
+    avg_ms = ms / n_launch
+    achieved = bytes_step / per_step_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_tend (tendency + predictor epilogue)",
+                "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "bytes_per_launch": int(bytes_step / per_step_launch),
+                "avg_launch_us": round(avg_ms * 1e3, 3),
+                "dominant_family_by_time": dom,
+                "family_ms_per_step": {k: round(v[1] / 60.0, 4) for k, v in prof.items()}}
+    cpu = None
+    if not args.no_cpu:
+        cv, cel = cpu_baseline(ic, args.cpu_steps)
+        cpu = {"value": round(cv, 6), "unit": "model-days/s", "cores": 1, "kind": "port",
+               "sample": f"first {args.cpu_steps} steps of the same 720x361 FP64 workload, "
+                         f"oracle/gd_oracle.c single thread ({cel:.1f} s)"}
+    lat = np.array(mixed["lat"]) if mixed["lat"] else np.array([np.nan])
+    lat64 = np.array(res["fp64"]["lat"]) if res["fp64"]["lat"] else np.array([np.nan])
+    out = {
+        "metric": METRIC,
+        "value": round(val["mixed"], 4),
+        "unit": "model-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["elapsed"] / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64",
+        "data": "synthetic (seeded balanced spherical-harmonic IC + 3 Gaussian mountains)",
+        "config": {"workload": "barotropic step 720x361 (0.5 deg), dt=120 s, 10 model days, "
+                               "csp2 x6, qcon, center_diff, diffusion 2.5e4, polar filter "
+                               f"cos(lat)/cos({FILTER_LAT0:g})",
+                   "grid": "720x361", "model_days_timed": round(days, 3),
+                   "precision": "mixed", "fp64_model_days_per_s": round(val["fp64"], 4),
+                   "mixed_over_fp64": round(val["mixed"] / val["fp64"], 3),
+                   "filtered_rows_per_hemisphere": len(mixed["cut"]),
+                   "parallelism": "single GPU, single run"},
+        "latency_s": {"runs": int(len(mixed["lat"])), "p50": float(np.percentile(lat, 50)),
+                      "p99": float(np.percentile(lat, 99)),
+                      "fp64_p50": float(np.percentile(lat64, 50)),
+                      "fp64_p99": float(np.percentile(lat64, 99))},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -47,7 +48,10 @@ struct gd_handle {
   std::vector<FiltTile> tiles;
   FiltTile *dtiles = nullptr;
   int kpad = 64, nchunk1 = 1, nchunk2 = 1;
-  double *dfcoef = nullptr, *dfs2 = nullptr, *dftab = nullptr;
+  double *dfcoef = nullptr, *dfs2 = nullptr;
+  void *dbasis = nullptr, *dbasis_t = nullptr;  // B[s][l] and Bt[l][s] in the compute type
+  void *dbasis64 = nullptr, *dbasis_t64 = nullptr;  // FP64 copies (aliases in FP64 mode)
+  int lpad = 64, spad = 128;
   uint32_t *ddone = nullptr;
   int32_t *dstep = nullptr;
   double *ddiag = nullptr;  // [nmem][cap][3]
@@ -95,7 +99,12 @@ Args base_args(const gd_handle *h) {
   a.tiles = h->dtiles;
   a.fcoef = h->dfcoef;
   a.fs2 = h->dfs2;
-  a.ftab = h->dftab;
+  a.basis = h->dbasis;
+  a.basis_t = h->dbasis_t;
+  a.basis64 = h->dbasis64;
+  a.basis_t64 = h->dbasis_t64;
+  a.lpad = h->lpad;
+  a.spad = h->spad;
   a.done = h->ddone;
   a.step_ctr = h->dstep;
   a.N = h->N;
@@ -223,6 +232,13 @@ hipError_t enqueue_step(gd_handle *h) {
 
 int build_step_graph(gd_handle *h) {
   if (h->graph || h->graph_failed) return GD_OK;
+  // GD_NO_GRAPH=1 launches every kernel eagerly (tracing tools that do not
+  // follow graph launches)
+  const char *ng = std::getenv("GD_NO_GRAPH");
+  if (ng && ng[0] == '1') {
+    h->graph_failed = true;
+    return GD_OK;
+  }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) { h->graph_failed = true; return GD_OK; }
@@ -287,18 +303,36 @@ void build_tables(gd_handle *h, std::vector<RowCoef> &rc) {
     if (g.full_flag[j]) add(CDGD, j, g.full_cut[j]);
   // tiles: (member, slot) pairs sorted by kept count so a tile's rows share
   // a slot range; ensemble members of one slot stay together.
-  struct Item { int kept, slot, member; };
+  // Tiles.  A row's arithmetic must not depend on its tile-mates, so that a
+  // run gives the same bits alone and batched: the projection mode is a
+  // property of the row (complement when more than half of the slots are
+  // kept), tiles hold rows of one mode, and slot ranges start on absolute
+  // 64-slot boundaries (kept mode at 0, complement mode at the smallest kept
+  // count rounded down), so every row sees the same k-grouping and chunking.
+  // Mass-continuity rows (dgd) form their own tiles: in mixed mode they run
+  // the FP64 GEMM so the filtered mass flux keeps FP64 accuracy.
+  struct Item { int cls, mode, kept, slot, member; };
   std::vector<Item> items;
   for (int s = 0; s < (int)h->slots.size(); ++s)
-    for (int m = 0; m < h->nmem; ++m) items.push_back({h->slots[s].kept, s, m});
+    for (int m = 0; m < h->nmem; ++m) {
+      const int kept = h->slots[s].kept;
+      items.push_back({h->slots[s].comp == CDGD ? 1 : 0, 2 * kept > N ? 1 : 0, kept, s, m});
+    }
   std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+    if (x.cls != y.cls) return x.cls < y.cls;
+    if (x.mode != y.mode) return x.mode < y.mode;
     return x.kept != y.kept ? x.kept < y.kept : (x.slot != y.slot ? x.slot < y.slot : x.member < y.member);
   });
   h->tiles.clear();
   int kmax_range = 1;
-  for (size_t b = 0; b < items.size(); b += kFiltRows) {
+  for (size_t b = 0; b < items.size();) {
     FiltTile t{};
-    t.nrows = (int)std::min<size_t>(kFiltRows, items.size() - b);
+    size_t e = b;
+    while (e < items.size() && e - b < (size_t)kFiltRows && items[e].cls == items[b].cls &&
+           items[e].mode == items[b].mode)
+      ++e;
+    t.nrows = (int)(e - b);
+    t.fp64 = h->prec == 0 || items[b].cls == 1;
     int kmin = N, kmax = 0;
     for (int r = 0; r < t.nrows; ++r) {
       t.member[r] = items[b + r].member;
@@ -306,13 +340,14 @@ void build_tables(gd_handle *h, std::vector<RowCoef> &rc) {
       kmin = std::min(kmin, items[b + r].kept);
       kmax = std::max(kmax, items[b + r].kept);
     }
-    if (kmax <= N - kmin) {
+    if (items[b].mode == 0) {
       t.complement = 0; t.lo = 0; t.hi = kmax;
     } else {
-      t.complement = 1; t.lo = kmin; t.hi = N;
+      t.complement = 1; t.lo = (kmin / kChunk) * kChunk; t.hi = N;
     }
     kmax_range = std::max(kmax_range, t.hi - t.lo);
     h->tiles.push_back(t);
+    b = e;
   }
   h->kpad = ((kmax_range + kChunk - 1) / kChunk) * kChunk;
   h->nchunk1 = h->kpad / kChunk;
@@ -416,8 +451,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   std::vector<RowCoef> rc;
   build_tables(h, rc);
   const size_t lds_need = std::max({tend_lds_bytes(h->N, h->cts), diffuse_lds_bytes(h->N),
-                                    filt_coef_lds_bytes(h->N, h->cts),
-                                    filt_apply_lds_bytes(h->N, h->kpad, h->cts)});
+                                    filt_coef_lds_bytes(), filt_apply_lds_bytes(h->kpad)});
   if (lds_need > prop.sharedMemPerBlock) {
     delete h;
     return fail(nullptr, GD_ERR_INVALID_ARG, "num_lon too large for the LDS of this device");
@@ -436,7 +470,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   const size_t per_member = al(fld * 8) * (NSLOT * NFLD + 1 + 2 + 2) +  // states, ghs, tb/tc dgd, div/vor
                             al(fld * h->cts) * 4 +                        // tb/tc du, dv
                             al((size_t)h->M * 3 * h->nchunk2 * 2 * 8) + al((size_t)h->M * 4 * 8) +
-                            al(std::max<size_t>(nslots, 1) * h->P * h->cts) +
+                            al(std::max<size_t>(nslots, 1) * h->P * 8) +
                             al(std::max<size_t>(nslots, 1) * 8) + al(8);
   h->hm.resize(h->nmem);
   for (int m = 0; m < h->nmem; ++m) {
@@ -455,7 +489,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
     mb.vor = (double *)take(fld * 8);
     mb.part = (double *)take((size_t)h->M * 3 * h->nchunk2 * 2 * 8);
     mb.dpart = (double *)take((size_t)h->M * 4 * 8);
-    mb.sx = take(std::max<size_t>(nslots, 1) * h->P * h->cts);
+    mb.sx = (double *)take(std::max<size_t>(nslots, 1) * h->P * 8);
     mb.s1 = (double *)take(std::max<size_t>(nslots, 1) * 8);
     mb.beta = (double *)take(8);
   }
@@ -467,26 +501,63 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->dtiles = (FiltTile *)dalloc(h, sizeof(FiltTile) * std::max<size_t>(h->tiles.size(), 1), e);
   h->dfcoef = (double *)dalloc(h, std::max<size_t>(h->tiles.size(), 1) * kFiltRows * h->kpad * 8, e);
   h->dfs2 = (double *)dalloc(h, std::max<size_t>(h->tiles.size(), 1) * h->nchunk1 * kFiltRows * 8, e);
-  h->dftab = (double *)dalloc(h, (size_t)2 * h->N * 8, e);
+  h->lpad = ((h->N + 63) / 64) * 64;
+  h->spad = h->lpad + 64;
+  h->dbasis = dalloc(h, (size_t)h->N * h->lpad * h->cts, e);
+  h->dbasis_t = dalloc(h, (size_t)h->N * h->spad * h->cts, e);
+  if (h->cts == 8) {
+    h->dbasis64 = h->dbasis;
+    h->dbasis_t64 = h->dbasis_t;
+  } else {
+    h->dbasis64 = dalloc(h, (size_t)h->N * h->lpad * 8, e);
+    h->dbasis_t64 = dalloc(h, (size_t)h->N * h->spad * 8, e);
+  }
   h->ddone = (uint32_t *)dalloc(h, sizeof(uint32_t) * h->nmem, e);
   h->dstep = (int32_t *)dalloc(h, sizeof(int32_t) * h->nmem, e);
   if (!h->ddiag || !h->dm || !h->dcoef || !h->dslots || !h->dtiles || !h->dfcoef || !h->dfs2 ||
-      !h->dftab || !h->ddone || !h->dstep) {
+      !h->dbasis || !h->dbasis_t || !h->dbasis64 || !h->dbasis_t64 || !h->ddone || !h->dstep) {
     gd_destroy(h);
     return fail(nullptr, GD_ERR_HIP, "device allocation failed");
   }
+  // FFTPACK5 real basis: slot 0 -> 1, slot 2k-1 -> cos(k l 2pi/N), slot 2k ->
+  // sin(k l 2pi/N) (for even N the last slot is cos(N/2 ...)); angles reduced
+  // mod N exactly before the FP64 cos/sin.
   std::vector<double> tab(2 * h->N);
   for (int q = 0; q < h->N; ++q) {
     const double ang = 2.0 * kPi * (double)q / (double)h->N;
     tab[q] = std::cos(ang);
     tab[h->N + q] = std::sin(ang);
   }
+  std::vector<double> bb((size_t)h->N * h->lpad, 0.0), bt((size_t)h->N * h->spad, 0.0);
+  for (int sl = 0; sl < h->N; ++sl)
+    for (int l = 0; l < h->N; ++l) {
+      double v;
+      if (sl == 0) {
+        v = 1.0;
+      } else {
+        const long k = (sl + 1) / 2, qq = (k * l) % h->N;
+        v = (sl & 1) ? tab[qq] : tab[h->N + qq];
+      }
+      bb[(size_t)sl * h->lpad + l] = v;
+      bt[(size_t)l * h->spad + sl] = v;
+    }
+  std::vector<float> bbf, btf;
+  if (h->cts == 4) {
+    bbf.assign(bb.begin(), bb.end());
+    btf.assign(bt.begin(), bt.end());
+  }
   hipMemcpyAsync(h->dm, h->hm.data(), sizeof(Member) * h->nmem, hipMemcpyHostToDevice, h->stream);
   hipMemcpyAsync(h->dcoef, rc.data(), sizeof(RowCoef) * h->M, hipMemcpyHostToDevice, h->stream);
   if (nslots) hipMemcpyAsync(h->dslots, h->slots.data(), sizeof(StageSlot) * nslots, hipMemcpyHostToDevice, h->stream);
   if (!h->tiles.empty())
     hipMemcpyAsync(h->dtiles, h->tiles.data(), sizeof(FiltTile) * h->tiles.size(), hipMemcpyHostToDevice, h->stream);
-  hipMemcpyAsync(h->dftab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, h->stream);
+  hipMemcpyAsync(h->dbasis64, bb.data(), bb.size() * 8, hipMemcpyHostToDevice, h->stream);
+  hipMemcpyAsync(h->dbasis_t64, bt.data(), bt.size() * 8, hipMemcpyHostToDevice, h->stream);
+  if (h->cts == 4) {
+    hipMemcpyAsync(h->dbasis, bbf.data(), bbf.size() * 4, hipMemcpyHostToDevice, h->stream);
+    hipMemcpyAsync(h->dbasis_t, btf.data(), btf.size() * 4, hipMemcpyHostToDevice, h->stream);
+    hipStreamSynchronize(h->stream);
+  }
   hipMemsetAsync(h->ddone, 0, sizeof(uint32_t) * h->nmem, h->stream);
   hipMemsetAsync(h->dstep, 0, sizeof(int32_t) * h->nmem, h->stream);
   e = hipStreamSynchronize(h->stream);
@@ -666,16 +737,11 @@ int gd_filter_rows(gd_handle *h, int32_t half, double *x, int64_t ld) {
   hipSetDevice(h->device);
   const int comp = half ? CDV : CDGD;
   const Member &mb = h->hm[0];
-  const size_t rowb = (size_t)h->P * h->cts;
-  std::vector<char> row(rowb);
+  const size_t rowb = (size_t)h->P * 8;
   for (size_t s = 0; s < h->slots.size(); ++s) {
     if (h->slots[s].comp != comp) continue;
-    const double *src = x + (size_t)h->slots[s].row * ld;
-    for (int i = 0; i < h->N; ++i) {
-      if (h->cts == 8) ((double *)row.data())[i] = src[i];
-      else ((float *)row.data())[i] = (float)src[i];
-    }
-    GD_HIP(h, hipMemcpy((char *)mb.sx + s * rowb, row.data(), rowb, hipMemcpyHostToDevice));
+    GD_HIP(h, hipMemcpy(mb.sx + s * h->P, x + (size_t)h->slots[s].row * ld, (size_t)h->N * 8,
+                        hipMemcpyHostToDevice));
   }
   Args a = base_args(h);
   a.phase = PH_P;
@@ -683,11 +749,10 @@ int gd_filter_rows(gd_handle *h, int32_t half, double *x, int64_t ld) {
   GD_HIP(h, hipStreamSynchronize(h->stream));
   for (size_t s = 0; s < h->slots.size(); ++s) {
     if (h->slots[s].comp != comp) continue;
-    GD_HIP(h, hipMemcpy(row.data(), (char *)mb.sx + s * rowb, rowb, hipMemcpyDeviceToHost));
-    double *dst = x + (size_t)h->slots[s].row * ld;
-    for (int i = 0; i < h->N; ++i)
-      dst[i] = h->cts == 8 ? ((double *)row.data())[i] : (double)((float *)row.data())[i];
+    GD_HIP(h, hipMemcpy(x + (size_t)h->slots[s].row * ld, mb.sx + s * h->P, (size_t)h->N * 8,
+                        hipMemcpyDeviceToHost));
   }
+  (void)rowb;
   return GD_OK;
 }
 

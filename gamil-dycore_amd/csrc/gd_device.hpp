@@ -50,7 +50,7 @@ struct Member {
   void *tc[3];               // Tc
   double *part;              // beta partials [M][3 comps][nchunk][2]
   double *dpart;             // diag partials [M][4]
-  void *sx;                  // staged raw filter rows [nslots][pitch] (compute type)
+  double *sx;                // staged raw filter rows [nslots][pitch]
   double *s1;                // [nslots] row inner product before filtering
   double *beta;              // [1] beta of the last integrator call
   double *div, *vor;         // history diagnostics
@@ -71,6 +71,8 @@ struct FiltTile {
   int32_t nrows;
   int32_t complement;
   int32_t lo, hi;              // slot range handled by the GEMM
+  int32_t fp64;                // FP64 GEMM (all tiles in FP64 mode; dgd tiles in mixed mode)
+  int32_t pad;
   int32_t member[kFiltRows];
   int32_t slot[kFiltRows];
 };
@@ -87,7 +89,11 @@ struct Args {
   const FiltTile *tiles;
   double *fcoef;               // filter coefficients [ntile][kFiltRows][kpad]
   double *fs2;                 // s2 partials [ntile][nchunk1][kFiltRows]
-  const double *ftab;          // cos table [N] then sin table [N] (double)
+  const void *basis;           // B[s][l], slot-major, pitch lpad (compute type)
+  const void *basis_t;         // Bt[l][s], longitude-major, pitch spad
+  const void *basis64;         // FP64 copies (same as basis/basis_t in FP64 mode)
+  const void *basis_t64;
+  int lpad, spad;
   uint32_t *done;              // per-member finish counters
   int32_t *step_ctr;
   int N, M, pitch, nmem;

@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
   const int st_dgd = cj.stage[CDGD];
   double s1_du = 0, s1_dv = 0, s1_dgd = 0;
   double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
-  CT *sx = (CT *)mb.sx;
+  double *sx = mb.sx;
 
   for (int i = tid; i < N; i += kBlock) {
     const int im = i == 0 ? N - 1 : i - 1, ip = i == N - 1 ? 0 : i + 1;
@@ -293,21 +293,21 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
     }
     // ---- filter staging or epilogue ----
     if (st_du >= 0) {
-      sx[(size_t)st_du * P + i] = (CT)du;
+      sx[(size_t)st_du * P + i] = du;
       s1_du += du * Ug[o + i];
     } else {
       phase_epilogue<CT>(a, mb, CDU, j, i, du, cj.fcos, p1[0], p2[0]);
     }
     if (has_v) {
       if (st_dv >= 0) {
-        sx[(size_t)st_dv * P + i] = (CT)dv;
+        sx[(size_t)st_dv * P + i] = dv;
         s1_dv += dv * Vg[o + i];
       } else {
         phase_epilogue<CT>(a, mb, CDV, j, i, dv, cj.hcos, p1[1], p2[1]);
       }
     }
     if (st_dgd >= 0) {
-      sx[(size_t)st_dgd * P + i] = (CT)dgd;
+      sx[(size_t)st_dgd * P + i] = dgd;
       s1_dgd += dgd * (gd[o + i] + ghs[o + i]);
     } else {
       phase_epilogue<CT>(a, mb, CDGD, j, i, dgd, cj.fcos, p1[2], p2[2]);
@@ -452,7 +452,7 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
   const int st_u = has_u ? cj.stage[CDU] : -1;
   const int st_v = has_v ? cj.stage[CDV] : -1;
   const int st_g = cj.stage[CDGD];
-  CT *sx = (CT *)mb.sx;
+  double *sx = mb.sx;
   const double kd = a.kdiff;
   double *Xgd = mb.st[SX][FGD], *Xu = mb.st[SX][Fu], *Xv = mb.st[SX][Fv];
   for (int i = tid; i < N; i += kBlock) {
@@ -484,16 +484,16 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
         vd = vd - (v0 - sv[i]) * cj.fcos / cj.hdlat2 * cj.hcos;
     }
     if (st_g >= 0)
-      sx[(size_t)st_g * P + i] = (CT)gdd;
+      sx[(size_t)st_g * P + i] = gdd;
     else
       Xgd[o + i] = gd[o + i] + kd * gdd;
     if (st_u >= 0)
-      sx[(size_t)st_u * P + i] = (CT)ud;
+      sx[(size_t)st_u * P + i] = ud;
     else
       Xu[o + i] = has_u ? su[N + i] + kd * ud : 0.0;
     if (has_v) {
       if (st_v >= 0)
-        sx[(size_t)st_v * P + i] = (CT)vd;
+        sx[(size_t)st_v * P + i] = vd;
       else
         Xv[o + i] = sv[N + i] + kd * vd;
     }
@@ -650,68 +650,71 @@ __device__ __forceinline__ double slot_weight(int N, int s) {
   return (s == 0 || ((N & 1) == 0 && s == N - 1)) ? 1.0 / N : 2.0 / N;
 }
 
-// weight row of a staged row (the inner product partner of the rescale)
-__device__ __forceinline__ double weight_val(const Args &a, const Member &mb, int comp, int row, int l) {
-  const size_t o = (size_t)row * a.pitch + l;
-  if (comp == CDU) return mb.st[a.s_in][FU][o];
-  if (comp == CDV) return mb.st[a.s_in][FV][o];
-  return mb.st[a.s_in][FGD][o] + mb.ghs[o];
-}
-
 // F1: coefficients C = X B^T over one tile x one chunk of 64 slots, masked
-// and weighted; also the s2 partial sum_s w_s C_x[s] C_w[s] of each row.
+// and weighted; also the s2 partial sum_s w_s C_x[s] C_w[s] of each row, which
+// equals <P x, w> because the projection P is symmetric.  Operands are staged
+// through LDS in 64-wide blocks: the tile's 8 staged rows and their 8 weight
+// rows (16 x 64) and the basis block Bt[l][s] (64 x 64).
 template <typename CT>
-__global__ void __launch_bounds__(kBlock) k_filter_coef(const Args a) {
-  const int wi = blockIdx.x;
-  if (wi >= a.ntiles) return;
-  const int tile = wi / a.nchunk1, chunk = wi % a.nchunk1;
-  const FiltTile ft = a.tiles[tile];
+__device__ __forceinline__ void filter_coef_body(const Args &a, const FiltTile *ft, int tile, int chunk,
+                                                 const CT *Bt, unsigned char *smem) {
   const int N = a.N, P = a.pitch;
-  const int slo = ft.lo + chunk * kChunk;
-  if (slo >= ft.hi) return;
-  extern __shared__ __align__(16) unsigned char smem[];
-  CT *ctab = (CT *)smem;
-  CT *stab = ctab + N;
-  constexpr int LB = 64;
-  CT *xs = stab + N;  // [16][LB]
+  const int slo = ft->lo + chunk * kChunk;
+  const int hi = ft->hi, lo = ft->lo, nrows = ft->nrows, comp_mode = ft->complement;
+  constexpr int LB = 64, BP = 65;
+  const double **rp = (const double **)smem;  // [16] source row, [16] second term (ghs)
+  const double **rq = rp + 16;
+  int *kept = (int *)(rq + 16);               // [8]
+  CT *xs = (CT *)(smem + 512);                // [16][BP]
+  CT *bs = xs + 16 * BP;                      // [64][BP]
   __shared__ double s2w[4][kFiltRows];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int q = tid; q < N; q += kBlock) {
-    ctab[q] = (CT)a.ftab[q];
-    stab[q] = (CT)a.ftab[N + q];
-  }
   const bool want_w = a.phase <= PH_T;
+  if (tid < 16) {
+    const int rr = tid & 7;
+    const double *p = nullptr, *q = nullptr;
+    if (rr < nrows) {
+      const Member &mb = a.mem[ft->member[rr]];
+      const StageSlot ss = a.slots[ft->slot[rr]];
+      if (tid < 8) {
+        p = mb.sx + (size_t)ft->slot[rr] * P;
+        kept[rr] = ss.kept;
+      } else if (want_w) {
+        const size_t o = (size_t)ss.row * P;
+        p = mb.st[a.s_in][ss.comp == CDU ? FU : (ss.comp == CDV ? FV : FGD)] + o;
+        q = ss.comp == CDGD ? mb.ghs + o : nullptr;
+      }
+    } else if (tid < 8) {
+      kept[rr] = 0;
+    }
+    rp[tid] = p;
+    rq[tid] = q;
+  }
   typename Mfma<CT>::acc_t acc = {0, 0, 0, 0};
   const int s_mine = slo + wave * 16 + (lane & 15);
-  const bool s_ok = s_mine < ft.hi;
+  const bool s_ok = s_mine < hi;
   for (int l0 = 0; l0 < N; l0 += LB) {
     __syncthreads();
     for (int e = tid; e < 16 * LB; e += kBlock) {
-      const int r = e / LB, l = l0 + e % LB;
-      CT v = (CT)0;
-      if (l < N) {
-        const int rr = r & 7;
-        if (rr < ft.nrows) {
-          const Member &mb = a.mem[ft.member[rr]];
-          const StageSlot ss = a.slots[ft.slot[rr]];
-          if (r < 8)
-            v = ((const CT *)mb.sx)[(size_t)ft.slot[rr] * P + l];
-          else if (want_w)
-            v = (CT)weight_val(a, mb, ss.comp, ss.row, l);
-        }
-      }
-      xs[e] = v;
+      const int r = e >> 6, l = l0 + (e & 63);
+      const double *p = rp[r], *q = rq[r];
+      double v = 0.0;
+      if (l < N && p) v = q ? p[l] + q[l] : p[l];
+      xs[r * BP + (e & 63)] = (CT)v;
+    }
+    for (int e = tid; e < LB * LB; e += kBlock) {
+      const int kk = e >> 6, c = e & 63, l = l0 + kk;
+      bs[kk * BP + c] = l < N ? Bt[(size_t)l * a.spad + slo + c] : (CT)0;
     }
     __syncthreads();
+#pragma unroll 4
     for (int kk = 0; kk < LB; kk += 4) {
       const int k = kk + (lane >> 4);
-      const CT av = xs[(lane & 15) * LB + k];
-      const CT bv = s_ok ? basis<CT>(ctab, stab, N, s_mine, l0 + k) : (CT)0;
-      acc = Mfma<CT>::run(av, bv, acc);
+      acc = Mfma<CT>::run(xs[(lane & 15) * BP + k], bs[k * BP + wave * 16 + (lane & 15)], acc);
     }
   }
   // acc[r] = C[acc_row(q, r)][slot s_mine]; weight row = staged row + 8
-  const int q = lane >> 4;
+  const int qd = lane >> 4;
   double part[4];
   double *fc = a.fcoef + (size_t)tile * kFiltRows * a.kpad;
   const double ws = s_ok ? slot_weight(N, s_mine) : 0.0;
@@ -724,88 +727,102 @@ __global__ void __launch_bounds__(kBlock) k_filter_coef(const Args a) {
       accw[r] = __shfl_xor(accd[r], 32, 64);  // FP32: row + 8 is 32 lanes away
   }
   for (int r = 0; r < 4; ++r) {
-    const int row = acc_row<CT>(q, r);  // 0..15
-    const double cx = accd[r];
-    const double cw = accw[r];
+    const int row = acc_row<CT>(qd, r);  // 0..15
     part[r] = 0.0;
-    if (row < 8 && row < ft.nrows && s_ok) {
-      const StageSlot ss = a.slots[ft.slot[row]];
-      const bool keep = ft.complement ? s_mine >= ss.kept : s_mine < ss.kept;
-      const double cval = keep ? ws * cx : 0.0;
-      fc[(size_t)row * a.kpad + (s_mine - ft.lo)] = cval;
-      part[r] = cval * cw;
-    } else if (row < 8 && s_mine - ft.lo < a.kpad && s_mine < ft.lo + a.kpad) {
-      fc[(size_t)row * a.kpad + (s_mine - ft.lo)] = 0.0;
+    if (row >= kFiltRows) continue;
+    const int sl = s_mine - lo;
+    if (sl >= a.kpad) continue;
+    double cval = 0.0;
+    if (row < nrows && s_ok) {
+      const bool keep = comp_mode ? s_mine >= kept[row] : s_mine < kept[row];
+      cval = keep ? ws * accd[r] : 0.0;
+      part[r] = cval * accw[r];
     }
+    fc[(size_t)row * a.kpad + sl] = cval;
   }
   for (int r = 0; r < 4; ++r) part[r] = group16_sum(part[r]);
   if ((lane & 15) == 0)
     for (int r = 0; r < 4; ++r) {
-      const int row = acc_row<CT>(q, r);
+      const int row = acc_row<CT>(qd, r);
       if (row < kFiltRows) s2w[wave][row] = part[r];
     }
   __syncthreads();
   if (tid < kFiltRows) {
-    const double s = ((s2w[0][tid] + s2w[1][tid]) + s2w[2][tid]) + s2w[3][tid];
-    a.fs2[((size_t)tile * a.nchunk1 + chunk) * kFiltRows + tid] = s;
+    const double sv = ((s2w[0][tid] + s2w[1][tid]) + s2w[2][tid]) + s2w[3][tid];
+    a.fs2[((size_t)tile * a.nchunk1 + chunk) * kFiltRows + tid] = sv;
   }
+}
+
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) k_filter_coef(const Args a) {
+  const int wi = blockIdx.x;
+  if (wi >= a.ntiles) return;
+  const int tile = wi / a.nchunk1, chunk = wi % a.nchunk1;
+  const FiltTile *ft = a.tiles + tile;
+  if (ft->lo + chunk * kChunk >= ft->hi) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (sizeof(CT) == 4 && ft->fp64)
+    filter_coef_body<double>(a, ft, tile, chunk, (const double *)a.basis_t64, smem);
+  else
+    filter_coef_body<CT>(a, ft, tile, chunk, (const CT *)a.basis_t, smem);
 }
 
 // F2: reconstruction Y = C' B over one tile x 64 columns, the rescale
 // tend*s1/s2 (dycore_mod.F90:213-217) and the phase epilogue.
-template <typename CT>
-__global__ void __launch_bounds__(kBlock) k_filter_apply(const Args a) {
-  const int wi = blockIdx.x;
-  if (wi >= a.ntiles) return;
-  const int tile = wi / a.nchunk2, chunk = wi % a.nchunk2;
-  const FiltTile ft = a.tiles[tile];
+template <typename CT, typename ST>
+__device__ __forceinline__ void filter_apply_body(const Args &a, const FiltTile *ft, int tile, int chunk,
+                                                  const CT *B, unsigned char *smem) {
   const int N = a.N, P = a.pitch;
-  extern __shared__ __align__(16) unsigned char smem[];
-  CT *ctab = (CT *)smem;
-  CT *stab = ctab + N;
-  CT *cs = stab + N;  // [8][kpad]
+  const int lo = ft->lo, hi = ft->hi, nrows = ft->nrows, comp_mode = ft->complement;
+  constexpr int LB = 64, BP = 65;
+  CT *bs = (CT *)smem;          // [64][BP] basis block B[s][l]
+  CT *cs = bs + LB * BP;        // [8][kpad]
   __shared__ double red[4][2][kFiltRows];
   __shared__ double s2row[kFiltRows];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ks = ft.hi - ft.lo;
-  for (int q = tid; q < N; q += kBlock) {
-    ctab[q] = (CT)a.ftab[q];
-    stab[q] = (CT)a.ftab[N + q];
-  }
+  const int ks = hi - lo;
   const double *fc = a.fcoef + (size_t)tile * kFiltRows * a.kpad;
   for (int e = tid; e < kFiltRows * a.kpad; e += kBlock) cs[e] = (CT)fc[e];
   if (tid < kFiltRows) {
-    double s = 0.0;
+    double sv = 0.0;
     const int nch = (ks + kChunk - 1) / kChunk;
-    for (int c = 0; c < nch; ++c) s += a.fs2[((size_t)tile * a.nchunk1 + c) * kFiltRows + tid];
-    s2row[tid] = s;
+    for (int c = 0; c < nch; ++c) sv += a.fs2[((size_t)tile * a.nchunk1 + c) * kFiltRows + tid];
+    s2row[tid] = sv;
+  }
+  const int col0 = chunk * kChunk;
+  const int col = col0 + wave * 16 + (lane & 15);
+  typename Mfma<CT>::acc_t acc = {0, 0, 0, 0};
+  for (int s0 = 0; s0 < ks; s0 += LB) {
+    __syncthreads();
+    for (int e = tid; e < LB * LB; e += kBlock) {
+      const int kk = e >> 6, c = e & 63, sidx = s0 + kk;
+      bs[kk * BP + c] = sidx < ks ? B[(size_t)(lo + sidx) * a.lpad + col0 + c] : (CT)0;
+    }
+    __syncthreads();
+    const int kend = ks - s0 < LB ? ((ks - s0 + 3) & ~3) : LB;
+    for (int kk = 0; kk < kend; kk += 4) {
+      const int k = kk + (lane >> 4);
+      const int row = lane & 15;
+      const CT av = (row < kFiltRows && s0 + k < ks) ? cs[row * a.kpad + s0 + k] : (CT)0;
+      acc = Mfma<CT>::run(av, bs[k * BP + wave * 16 + (lane & 15)], acc);
+    }
   }
   __syncthreads();
-  const int col = chunk * kChunk + wave * 16 + (lane & 15);
-  typename Mfma<CT>::acc_t acc = {0, 0, 0, 0};
-  const int kp4 = (ks + 3) & ~3;
-  for (int s0 = 0; s0 < kp4; s0 += 4) {
-    const int k = s0 + (lane >> 4);
-    const int row = lane & 15;
-    const CT av = (row < 8 && k < ks) ? cs[row * a.kpad + k] : (CT)0;
-    const CT bv = (k < ks) ? basis<CT>(ctab, stab, N, ft.lo + k, col) : (CT)0;
-    acc = Mfma<CT>::run(av, bv, acc);
-  }
-  const int q = lane >> 4;
+  const int qd = lane >> 4;
   double p1[4] = {0, 0, 0, 0}, p2[4] = {0, 0, 0, 0};
   if (col < N) {
     for (int r = 0; r < 4; ++r) {
-      const int row = acc_row<CT>(q, r);
-      if (row >= ft.nrows) continue;
-      const Member &mb = a.mem[ft.member[row]];
-      const int slot = ft.slot[row];
+      const int row = acc_row<CT>(qd, r);
+      if (row >= nrows) continue;
+      const Member &mb = a.mem[ft->member[row]];
+      const int slot = ft->slot[row];
       const StageSlot ss = a.slots[slot];
-      const double x = (double)((const CT *)mb.sx)[(size_t)slot * P + col];
-      const double y = ft.complement ? x - (double)acc[r] : (double)acc[r];
+      const double x = mb.sx[(size_t)slot * P + col];
+      const double y = comp_mode ? x - (double)acc[r] : (double)acc[r];
       double T;
       if (a.phase <= PH_T) {
         const double s1 = mb.s1[slot];
-        const double s2 = ft.complement ? s1 - s2row[row] : s2row[row];
+        const double s2 = comp_mode ? s1 - s2row[row] : s2row[row];
         T = fabs(s1) > 1.0e-16 ? y * s1 / s2 : x;
       } else {
         T = y;
@@ -813,7 +830,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_apply(const Args a) {
       const size_t o = (size_t)ss.row * P + col;
       if (a.phase <= PH_T) {
         const double w = ss.comp == CDV ? a.coef[ss.row].hcos : a.coef[ss.row].fcos;
-        phase_epilogue<CT>(a, mb, ss.comp, ss.row, col, T, w, p1[r], p2[r]);
+        phase_epilogue<ST>(a, mb, ss.comp, ss.row, col, T, w, p1[r], p2[r]);
       } else if (a.phase == PH_D) {
         // ordinary_diffusion update of the filtered component
         const double *sgd = mb.st[a.s_in][FGD];
@@ -830,7 +847,7 @@ __global__ void __launch_bounds__(kBlock) k_filter_apply(const Args a) {
           mb.st[SX][Fv][o] = v + a.kdiff * T;
         }
       } else {  // PH_P: projection only, result back into the staging row
-        ((CT *)mb.sx)[(size_t)slot * P + col] = (CT)T;
+        mb.sx[(size_t)slot * P + col] = T;
       }
     }
   }
@@ -841,21 +858,34 @@ __global__ void __launch_bounds__(kBlock) k_filter_apply(const Args a) {
     }
     if ((lane & 15) == 0)
       for (int r = 0; r < 4; ++r) {
-        const int row = acc_row<CT>(q, r);
+        const int row = acc_row<CT>(qd, r);
         if (row < kFiltRows) {
           red[wave][0][row] = p1[r];
           red[wave][1][row] = p2[r];
         }
       }
     __syncthreads();
-    if (tid < ft.nrows) {
-      const Member &mb = a.mem[ft.member[tid]];
-      const StageSlot ss = a.slots[ft.slot[tid]];
+    if (tid < nrows) {
+      const Member &mb = a.mem[ft->member[tid]];
+      const StageSlot ss = a.slots[ft->slot[tid]];
       double *pp = mb.part + ((size_t)(ss.row * 3 + ss.comp) * a.nchunk_row + chunk) * 2;
       pp[0] = ((red[0][0][tid] + red[1][0][tid]) + red[2][0][tid]) + red[3][0][tid];
       pp[1] = ((red[0][1][tid] + red[1][1][tid]) + red[2][1][tid]) + red[3][1][tid];
     }
   }
+}
+
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) k_filter_apply(const Args a) {
+  const int wi = blockIdx.x;
+  if (wi >= a.ntiles) return;
+  const int tile = wi / a.nchunk2, chunk = wi % a.nchunk2;
+  const FiltTile *ft = a.tiles + tile;
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (sizeof(CT) == 4 && ft->fp64)
+    filter_apply_body<double, CT>(a, ft, tile, chunk, (const double *)a.basis64, smem);
+  else
+    filter_apply_body<CT, CT>(a, ft, tile, chunk, (const CT *)a.basis, smem);
 }
 
 // MFMA operand-layout check used by the tests: D = A B for 16x4 A, 4x16 B.
@@ -955,18 +985,18 @@ hipError_t configure_kernels(int max_lds) {
 hipError_t launch_filter_coef(const Args &a, int prec, hipStream_t s) {
   dim3 g(a.ntiles), b(kBlock);
   if (prec == 0)
-    hipLaunchKernelGGL(k_filter_coef<double>, g, b, filt_coef_lds_bytes(a.N, 8), s, a);
+    hipLaunchKernelGGL(k_filter_coef<double>, g, b, filt_coef_lds_bytes(), s, a);
   else
-    hipLaunchKernelGGL(k_filter_coef<float>, g, b, filt_coef_lds_bytes(a.N, 4), s, a);
+    hipLaunchKernelGGL(k_filter_coef<float>, g, b, filt_coef_lds_bytes(), s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_filter_apply(const Args &a, int prec, hipStream_t s) {
   dim3 g(a.ntiles), b(kBlock);
   if (prec == 0)
-    hipLaunchKernelGGL(k_filter_apply<double>, g, b, filt_apply_lds_bytes(a.N, a.kpad, 8), s, a);
+    hipLaunchKernelGGL(k_filter_apply<double>, g, b, filt_apply_lds_bytes(a.kpad), s, a);
   else
-    hipLaunchKernelGGL(k_filter_apply<float>, g, b, filt_apply_lds_bytes(a.N, a.kpad, 4), s, a);
+    hipLaunchKernelGGL(k_filter_apply<float>, g, b, filt_apply_lds_bytes(a.kpad), s, a);
   return hipGetLastError();
 }
 

@@ -20,13 +20,13 @@ __host__ __device__ inline size_t tend_lds_bytes(int N, size_t cts) {
 __host__ __device__ inline size_t diffuse_lds_bytes(int N) {
   return (size_t)10 * N * 8 + 16 * sizeof(double);
 }
-// k_filter_coef: cos/sin tables + a 16 x 64 operand block.
-__host__ __device__ inline size_t filt_coef_lds_bytes(int N, size_t cts) {
-  return ((size_t)2 * N + 16 * 64) * cts;
-}
-// k_filter_apply: cos/sin tables + the tile's 8 x kpad coefficients.
-__host__ __device__ inline size_t filt_apply_lds_bytes(int N, int kpad, size_t cts) {
-  return ((size_t)2 * N + (size_t)kFiltRows * kpad) * cts;
+// k_filter_coef: 512 bytes of row pointers, a 16 x 64 operand block and a
+// 64 x 64 basis block (pitch 65), sized for FP64 operands.
+__host__ __device__ inline size_t filt_coef_lds_bytes() { return 512 + (size_t)(16 + 64) * 65 * 8; }
+// k_filter_apply: a 64 x 64 basis block (pitch 65) + the tile's 8 x kpad
+// coefficients, sized for FP64 operands.
+__host__ __device__ inline size_t filt_apply_lds_bytes(int kpad) {
+  return ((size_t)64 * 65 + (size_t)kFiltRows * kpad) * 8;
 }
 
 // prec: 0 = FP64 plain, 1 = mixed (FP32 compute type)

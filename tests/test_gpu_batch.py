@@ -1,0 +1,128 @@
+"""Mixed precision against FP64 and ensemble batching on the device.
+
+Mixed mode evaluates the momentum stencils and the momentum filter rows in
+FP32 while the state update, mass-flux divergence (and its filter rows), pole
+sums and inner products stay FP64.  Tolerances: one FP32 operator
+evaluation 5e-4 relative to the field maximum; the mass budget of a mixed run
+tracks the FP64 run to 1e-13 relative per day.
+"""
+import numpy as np
+import pytest
+
+from conftest import l2_rel, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(gd, N, M, **kw):
+    base = dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=1e5,
+                zonal_tend_filter_cutoff_wavenumber=gd.cutoff_from_lat0(N, M, 75.0))
+    base.update(kw)
+    return base
+
+
+def test_mixed_space_operators_vs_oracle(gd, oracle_mod):
+    N, M = 180, 91
+    u, v, h, ghs = gd.ic_balanced(N, M, seed=5, nmax=12, n_mountains=3, mountain_seed=9)
+    cut = gd.cutoff_from_lat0(N, M, 75.0)
+    o = oracle_mod.Oracle(N, M, 480.0, cutoff=cut)
+    o.set_initial_state(u, v, h, ghs)
+    o.start()
+    with gd.Dycore(N, M, 480.0, zonal_tend_filter_cutoff_wavenumber=cut, precision="mixed") as d:
+        d.set_state(u, v, h, ghs)
+        for pname, pid in (("all", 0), ("fast", 1), ("slow", 2)):
+            o.space_operators(o.old, 0, pid)
+            du, dv, dgd = d.space_operators(pname)
+            # FP32 tendencies: du/dv are small differences of nearly balanced
+            # Coriolis and pressure-gradient terms, and next to the poles the
+            # zonal advection u2*V(i+1) - u1*V(i-1) cancels across neighbours
+            # 480 m apart; both enlarge FP32 rounding ~1000x relative to the
+            # largest tendency
+            assert rel_err(du, o.tend(0, oracle_mod.T_DU)) < 5e-4, pname
+            assert rel_err(dv, o.tend(0, oracle_mod.T_DV)) < 5e-4, pname
+            # mass continuity is FP64 in mixed mode
+            if pname != "slow":
+                assert rel_err(dgd, o.tend(0, oracle_mod.T_DGD)) < 1e-12, pname
+
+
+@pytest.mark.parametrize("N,M,dt,steps", [(360, 181, 240.0, 360)])
+def test_mixed_one_day_vs_fp64(gd, N, M, dt, steps):
+    """1 day at 360x181: height l2 within 1e-6 of FP64, mass budget identical to 1e-13."""
+    ic = gd.ic_balanced(N, M, seed=21, nmax=20)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with gd.Dycore(N, M, dt, precision=prec, **_cfg(gd, N, M)) as d:
+            d.set_state(*ic)
+            m0, _ = d.diag()
+            mass, energy, beta, _ = d.step(steps)
+            out[prec] = (d.get_state(), mass[:, 0], energy[:, 0], m0[0])
+    (u64, v64, h64), mass64, en64, m064 = out["fp64"]
+    (u32, v32, h32), mass32, en32, m032 = out["mixed"]
+    assert m064 == m032
+    assert l2_rel(h32, h64) < 1e-6
+    assert l2_rel(u32, u64) < 1e-4
+    assert abs(mass32[-1] - mass64[-1]) <= 1e-13 * abs(mass64[-1])
+    assert np.all(np.abs(en32 - en64) <= 1e-8 * np.abs(en64))
+
+
+def test_ensemble_members_match_single_runs_bitwise(gd):
+    """Each member of a batched ensemble equals its own single run, bit for bit."""
+    N, M, dt = 180, 91, 480.0
+    cfg = _cfg(gd, N, M, subcycles=4)
+    ics = [gd.ic_balanced(N, M, seed=3, nmax=10, pert_seed=100 + k, pert_nmax=10, pert_rms=5.0)
+           for k in range(3)]
+    singles = []
+    for ic in ics:
+        with gd.Dycore(N, M, dt, **cfg) as d:
+            d.set_state(*ic)
+            mass, energy, beta, _ = d.step(20)
+            singles.append((d.get_state(), mass[:, 0], energy[:, 0], beta[:, 0]))
+    with gd.Dycore(N, M, dt, members=3, **cfg) as e:
+        for k, ic in enumerate(ics):
+            e.set_state(*ic, member=k)
+        mass, energy, beta, _ = e.step(20)
+        for k in range(3):
+            (u, v, h) = e.get_state(member=k)
+            (us, vs, hs), ms, es, bs = singles[k]
+            np.testing.assert_array_equal(h, hs)
+            np.testing.assert_array_equal(u, us)
+            np.testing.assert_array_equal(v, vs)
+            np.testing.assert_array_equal(mass[:, k], ms)
+            np.testing.assert_array_equal(energy[:, k], es)
+            np.testing.assert_array_equal(beta[:, k], bs)
+    # the members differ from each other
+    assert np.abs(singles[0][0][2] - singles[1][0][2]).max() > 0
+
+
+def test_nan_guard_reports_step(gd):
+    """An absurd time step drives the run to NaN; gd_step reports the first NaN step
+    (diag_mod.F90:79-87 stops the reference with an error there)."""
+    N, M = 72, 37
+    ic = gd.ic_balanced(N, M, seed=2, nmax=6)
+    with gd.Dycore(N, M, 2.0e5, split_scheme="none", use_zonal_tend_filter=False) as d:
+        d.set_state(*ic)
+        mass, energy, beta, nan_step = d.step(200, raise_on_nan=False)
+        assert nan_step >= 1
+        assert np.all(np.isfinite(mass[: nan_step - 1, 0]))
+        assert not np.isfinite(mass[nan_step - 1, 0]) or not np.isfinite(energy[nan_step - 1, 0])
+        with pytest.raises(gd.DycoreError) as ei:
+            d.step(5)
+        assert ei.value.code == gd.GD_ERR_NAN
+
+
+def test_rejects_bad_state_and_config(gd):
+    N, M = 72, 37
+    u, v, h, ghs = gd.ic_balanced(N, M, seed=2, nmax=6)
+    with gd.Dycore(N, M, 600.0) as d:
+        bad = u.copy()
+        bad[0, 3] = 1.0  # u must vanish on the pole rows
+        with pytest.raises(gd.DycoreError):
+            d.set_state(bad, v, h, ghs)
+        with pytest.raises(gd.DycoreError):
+            d.set_state(u, v, -h, ghs)
+    with pytest.raises(gd.DycoreError):
+        gd.Dycore(N, M, 600.0, split_scheme="isp")
+    with pytest.raises(gd.DycoreError):
+        gd.Dycore(N, M, 600.0, zonal_tend_filter_cutoff_wavenumber=[0, 4])
+    with pytest.raises(gd.DycoreError):
+        gd.Dycore(N, M, 600.0, zonal_tend_filter_cutoff_wavenumber=[N // 2])
