@@ -57,7 +57,7 @@ EXPORTS = (
     "gd_get_state", "gd_get_diag_fields", "gd_space_operators", "gd_predict_correct",
     "gd_ordinary_diffusion", "gd_filter_rows", "gd_filter_table", "gd_ic_balanced",
     "gd_ic_rossby_haurwitz", "gd_ic_mountain_zonal_flow", "gd_profile_kernels",
-    "gd_device_bytes", "gd_selftest_mfma",
+    "gd_device_bytes", "gd_selftest_mfma", "gd_cutoff_from_lat0",
 )
 
 _lib = None
@@ -95,6 +95,7 @@ def lib():
     L.gd_ic_mountain_zonal_flow.argtypes = [C.c_int32, C.c_int32, dp, dp, dp, dp, i64]
     L.gd_profile_kernels.argtypes = [vp, C.c_int32, C.c_int32, ip, C.POINTER(C.c_int64), dp,
                                      C.c_char_p, C.c_int32]
+    L.gd_cutoff_from_lat0.argtypes = [C.c_int32, C.c_int32, C.c_double, ip, C.c_int32]
     L.gd_device_bytes.restype = C.c_int64
     L.gd_device_bytes.argtypes = [vp]
     L.gd_selftest_mfma.argtypes = [dp, dp, dp, C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -145,20 +146,14 @@ def ic_mountain_zonal_flow(num_lon, num_lat):
 
 def cutoff_from_lat0(num_lon, num_lat, lat0_deg):
     """Polar filter table whose kept wavenumber falls toward the poles as
-    cos(lat)/cos(lat0): entry k (k = 1.. from each pole) drives full row 1+k,
-    c_k = floor(num_lon/2 * cos(lat_{1+k}) / cos(lat0)) for |lat| > lat0."""
-    N, M = int(num_lon), int(num_lat)
-    dlat = np.pi / (M - 1)
-    c0 = np.cos(np.deg2rad(lat0_deg))
-    out = []
-    for k in range(1, M):
-        lat = -0.5 * np.pi + k * dlat  # full row 1+k (1-based) from the south pole
-        if abs(np.rad2deg(lat)) <= lat0_deg:
-            break
-        c = int(np.floor(N / 2 * np.cos(lat) / c0))
-        c = max(1, min(c, N // 2 - 1))
-        out.append(c)
-    return out
+    cos(lat)/cos(lat0) (gd_cutoff_from_lat0 in the C ABI)."""
+    cap = int(num_lat)
+    out = np.zeros(cap, np.int32)
+    n = lib().gd_cutoff_from_lat0(int(num_lon), int(num_lat), float(lat0_deg),
+                                  out.ctypes.data_as(C.POINTER(C.c_int32)), cap)
+    if n < 0:
+        raise DycoreError(GD_ERR_INVALID_ARG, "bad arguments to cutoff_from_lat0")
+    return [int(c) for c in out[:n]]
 
 
 # ---------------------------------------------------------------------------

@@ -112,6 +112,7 @@ Args base_args(const gd_handle *h) {
   a.pitch = h->P;
   a.nmem = h->nmem;
   a.m0 = 0;
+  a.nseg = (h->N + kBlock - 1) / kBlock;
   a.ntiles = h->nmem * h->M;
   a.nslots = (int)h->slots.size();
   a.nftiles = (int)h->tiles.size();
@@ -161,6 +162,8 @@ hipError_t enqueue_call(gd_handle *h, double dt, int pass, int s0, int s_out, in
   a.dt = dt;
   a.hdt = dt * 0.5;
   hipError_t e;
+  const int row_tiles = h->nmem * h->M;
+  a.ntiles = row_tiles * a.nseg;  // (member, row, segment) tiles
   // first predictor: T = L(S0), S1 = S0 + dt/2 T
   a.phase = PH_A; a.s_in = s0; a.s0 = s0; a.s_out = SP1;
   e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, s0 == SX, h->stream); });
@@ -177,6 +180,8 @@ hipError_t enqueue_call(gd_handle *h, double dt, int pass, int s0, int s_out, in
   if (e != hipSuccess) return e;
   if ((e = enqueue_filter(h, a)) != hipSuccess) return e;
   a.s_out = s_out;
+  e = run(h, "beta", [&] { return launch_beta(a, h->nmem, h->stream); });
+  if (e != hipSuccess) return e;
   return run(h, "correct", [&] { return launch_correct(a, h->prec, h->stream); });
 }
 
@@ -450,7 +455,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   }
   std::vector<RowCoef> rc;
   build_tables(h, rc);
-  const size_t lds_need = std::max({tend_lds_bytes(h->N, h->cts), diffuse_lds_bytes(h->N),
+  const size_t lds_need = std::max({diffuse_lds_bytes(h->N),
                                     filt_coef_lds_bytes(), filt_apply_lds_bytes(h->kpad)});
   if (lds_need > prop.sharedMemPerBlock) {
     delete h;
@@ -697,6 +702,7 @@ int gd_space_operators(gd_handle *h, int32_t member, int32_t pass, double *du, d
   a.phase = PH_T;
   a.s_in = SX;
   a.s0 = SX;
+  a.ntiles = h->nmem * h->M * a.nseg;
   GD_HIP(h, launch_tend(a, h->prec, pass, true, h->stream));
   GD_HIP(h, enqueue_filter(h, a));
   const Member &mb = h->hm[member];

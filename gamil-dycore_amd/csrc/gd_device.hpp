@@ -98,6 +98,7 @@ struct Args {
   int32_t *step_ctr;
   int N, M, pitch, nmem;
   int m0;                      // first member of a row launch
+  int nseg;                    // longitude segments of kBlock points per row
   int ntiles;                  // work items of this launch
   int in_full;                 // k_diffuse: read stored u, v of s_in
   int nslots, nftiles, kpad, nchunk1, nchunk2, nchunk_row;

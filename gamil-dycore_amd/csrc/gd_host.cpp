@@ -1,5 +1,6 @@
 // gd_host.cpp -- one-time host setup: geometry/coefficients/filter table and
 // the initial-condition generators.  Nothing here runs inside the time step.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -400,4 +401,21 @@ extern "C" int gd_ic_mountain_zonal_flow(int32_t N, int32_t M, double *u, double
     for (int j = 0; j < M - 1; ++j)
       for (int i = 0; i < N; ++i) v[(size_t)j * ld + i] = 0.0;
   return GD_OK;
+}
+
+extern "C" int gd_cutoff_from_lat0(int32_t N, int32_t M, double lat0_deg, int32_t *cutoff, int32_t cap) {
+  using namespace gd;
+  if (N < 8 || M < 5 || !(lat0_deg > 0.0) || !(lat0_deg < 90.0)) return -1;
+  const double dlat = kPi / (M - 1);
+  const double c0 = std::cos(lat0_deg * kPi / 180.0);
+  int n = 0;
+  for (int k = 1; k < M; ++k) {
+    const double lat = -0.5 * kPi + k * dlat;  // full row 1+k from the south pole
+    if (std::fabs(lat * 180.0 / kPi) <= lat0_deg + 1e-9) break;  // rows strictly poleward of lat0
+    int c = (int)std::floor(N / 2.0 * std::cos(lat) / c0);
+    c = std::max(1, std::min(c, N / 2 - 1));
+    if (cutoff && n < cap) cutoff[n] = c;
+    ++n;
+  }
+  return n;
 }

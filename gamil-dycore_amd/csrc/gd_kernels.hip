@@ -101,59 +101,79 @@ __device__ __forceinline__ void phase_epilogue(const Args &a, const Member &mb, 
 // tendency (space_operators, dycore_mod.F90:184-598) + predictor epilogue
 // ---------------------------------------------------------------------------
 
+// Tile = (member, full row j, segment of kBlock longitudes); one thread per
+// longitude.  LDS rows cover the segment plus a halo of two points so the
+// zonal neighbours i-1, i+1 wrap periodically.
 template <typename CT, int PASS, bool IN_FULL>
 __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
-  const int N = a.N, M = a.M, P = a.pitch;
-  const int m = a.m0 + t / M, j = t % M;
+  const int N = a.N, M = a.M, P = a.pitch, nseg = a.nseg;
+  const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
+  const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
+  const int m = a.m0 + mloc;
+  const int i0 = g * kBlock, W = min(N - i0, kBlock);
+  constexpr int LW = kBlock + 4;  // LDS index li = i - i0 + 2
+  __shared__ double sG[4][LW];    // G rows j-1..j+2, points i0-1 .. i0+W+1
+  __shared__ CT su[2][LW];        // u rows j, j+1, points i0-1 .. i0+W
+  __shared__ CT sv[3][LW];        // v half rows j-1, j, j+1, points i0 .. i0+W
+  __shared__ double red[16];
   const Member &mb = a.mem[m];
-  extern __shared__ __align__(16) unsigned char smem[];
-  double *sG = (double *)smem;        // G rows j-1 .. j+2
-  CT *su = (CT *)(sG + 4 * N);        // u rows j, j+1
-  CT *sv = su + 2 * N;                // v half rows j-1, j, j+1
-  double *red = (double *)(smem + tend_lds_bytes(N, sizeof(CT)) - 16 * sizeof(double));
   const int tid = threadIdx.x;
   const double *gd = mb.st[a.s_in][FGD];
   const double *Ug = mb.st[a.s_in][FU];
   const double *Vg = mb.st[a.s_in][FV];
   const double *ghs = mb.ghs;
+  auto wrap = [N](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
 
+#pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int row = j - 1 + r;
-    if (row < 0 || row > M - 1) continue;
-    for (int i = tid; i < N; i += kBlock) sG[r * N + i] = sqrt(gd[(size_t)row * P + i]);
+    const bool ok = row >= 0 && row <= M - 1;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int e = tid + k * kBlock;  // point i0-1+e
+      if (ok && e < W + 3) sG[r][e + 1] = sqrt(gd[(size_t)row * P + wrap(i0 - 1 + e)]);
+    }
   }
   __syncthreads();
+#pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int row = j + r;
-    if (row > M - 1) continue;
-    for (int i = tid; i < N; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int e = tid + k * kBlock;  // point i0-1+e, li = e+1
+      if (row > M - 1 || e >= W + 2) continue;
+      const int i = wrap(i0 - 1 + e);
       CT val;
       if (IN_FULL) {
         val = (CT)mb.st[a.s_in][Fu][(size_t)row * P + i];
       } else if (row == 0 || row == M - 1) {
         val = (CT)0;
       } else {
-        const int ip = i + 1 == N ? 0 : i + 1;
         // update_state: u = U * 2 / (G(i) + G(i+1))  (dycore_mod.F90:636-640)
-        val = ((CT)Ug[(size_t)row * P + i] * (CT)2) / ((CT)sG[(r + 1) * N + i] + (CT)sG[(r + 1) * N + ip]);
+        val = ((CT)Ug[(size_t)row * P + i] * (CT)2) / ((CT)sG[r + 1][e + 1] + (CT)sG[r + 1][e + 2]);
       }
-      su[r * N + i] = val;
+      su[r][e + 1] = val;
     }
   }
+#pragma unroll
   for (int r = 0; r < 3; ++r) {
     const int hrow = j - 1 + r;
     const bool ok = hrow >= 0 && hrow <= M - 2;
-    for (int i = tid; i < N; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int e = tid + k * kBlock;  // point i0+e, li = e+2
+      if (e > W) continue;
       CT val = (CT)0;
       if (ok) {
+        const int i = wrap(i0 + e);
         if (IN_FULL)
           val = (CT)mb.st[a.s_in][Fv][(size_t)hrow * P + i];
         else  // v = V * 2 / (G(j) + G(j+1))  (dycore_mod.F90:641-645)
-          val = ((CT)Vg[(size_t)hrow * P + i] * (CT)2) / ((CT)sG[r * N + i] + (CT)sG[(r + 1) * N + i]);
+          val = ((CT)Vg[(size_t)hrow * P + i] * (CT)2) / ((CT)sG[r][e + 2] + (CT)sG[r + 1][e + 2]);
       }
-      sv[r * N + i] = val;
+      sv[r][e + 2] = val;
     }
   }
   __syncthreads();
@@ -169,34 +189,34 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
   const CT bl = (CT)a.beta_lon, bt = (CT)a.beta_lat;
   const bool upwind = a.upwind != 0;
 
-  // pole mass flux sums (dycore_mod.F90:572-596)
+  // pole mass flux sums over the whole ring (dycore_mod.F90:572-596); every
+  // segment of a pole row forms the same sum in the same order
   double pole = 0.0;
   if (PASS != 2 && (j == 0 || j == M - 1)) {
     double acc = 0.0;
-    if (j == 0) {
-      for (int i = tid; i < N; i += kBlock) acc += (sG[N + i] + sG[2 * N + i]) * Vg[i];
-    } else {
-      for (int i = tid; i < N; i += kBlock) acc += (sG[N + i] + sG[i]) * Vg[(size_t)(M - 2) * P + i];
-    }
-    double s = block_sum(acc, red);
-    if (j == M - 1) s = -s;
-    pole = s * 2.0 / N / kRadius / a.dlat;
+    const int jn = j == 0 ? 1 : M - 2, hr = j == 0 ? 0 : M - 2;
+    for (int i = tid; i < N; i += kBlock)
+      acc += (sqrt(gd[(size_t)j * P + i]) + sqrt(gd[(size_t)jn * P + i])) * Vg[(size_t)hr * P + i];
+    double sum = block_sum(acc, red);
+    if (j == M - 1) sum = -sum;
+    pole = sum * 2.0 / N / kRadius / a.dlat;
   }
 
   const int st_du = has_u ? cj.stage[CDU] : -1;
   const int st_dv = has_v ? cj.stage[CDV] : -1;
   const int st_dgd = cj.stage[CDGD];
-  double s1_du = 0, s1_dv = 0, s1_dgd = 0;
   double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
   double *sx = mb.sx;
 
-  for (int i = tid; i < N; i += kBlock) {
+  if (tid < W) {
+    const int li = tid + 2;
+    const int i = i0 + tid;
     const int im = i == 0 ? N - 1 : i - 1, ip = i == N - 1 ? 0 : i + 1;
     const size_t o = (size_t)j * P;
     // ---- u equation (full row j) ----
     double du = 0.0;
     if (has_u) {
-      const CT u0 = su[i], um = su[im], up = su[ip];
+      const CT u0 = su[0][li], um = su[0][li - 1], up = su[0][li + 1];
       CT ual = 0, uat = 0, fv = 0, upgf = 0;
       if (PASS != 1) {
         const CT U0 = (CT)Ug[o + i], Um = (CT)Ug[o + im], Up = (CT)Ug[o + ip];
@@ -207,8 +227,8 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
         else
           ual = kl * (u2 * (U0 + Up) - bl * fabs(u2) * (Up - U0) - u1 * (U0 + Um) +
                       bl * fabs(u1) * (U0 - Um) - (u2 - u1) * U0);
-        const CT v1 = (sv[i] + sv[ip]) * (CT)hc_m;
-        const CT v2 = (sv[N + i] + sv[N + ip]) * (CT)hc_0;
+        const CT v1 = (sv[0][li] + sv[0][li + 1]) * (CT)hc_m;
+        const CT v2 = (sv[1][li] + sv[1][li + 1]) * (CT)hc_0;
         const CT Ujp = (CT)Ug[o + P + i], Ujm = (CT)Ug[o - P + i];
         const CT kt = (CT)cj.k_uadv_lat;
         if (!upwind)
@@ -222,7 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
         const CT V00 = (CT)Vg[o + i], V01 = (CT)Vg[o + ip];
         fv = (CT)0.25 * ((CT)cj.f + (CT)cj.c * u0) * ((CT)cj.c1 * (Vm0 + Vm1) + (CT)cj.c2 * (V00 + V01));
         const double dphi = gd[o + ip] + ghs[o + ip] - gd[o + i] - ghs[o + i];
-        upgf = (CT)0.5 * ((CT)sG[N + i] + (CT)sG[N + ip]) / (CT)cj.fdlon * (CT)dphi;
+        upgf = (CT)0.5 * ((CT)sG[1][li] + (CT)sG[1][li + 1]) / (CT)cj.fdlon * (CT)dphi;
       }
       CT d;
       if (PASS == 0)
@@ -238,8 +258,8 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
     if (has_v) {
       CT val = 0, vat = 0, fu = 0, vpgf = 0;
       if (PASS != 1) {
-        const CT u1 = su[im] + su[N + im];
-        const CT u2 = su[i] + su[N + i];
+        const CT u1 = su[0][li - 1] + su[1][li - 1];
+        const CT u2 = su[0][li] + su[1][li];
         const CT V0 = (CT)Vg[o + i], Vm = (CT)Vg[o + im], Vp = (CT)Vg[o + ip];
         const CT kl = (CT)cj.k_vadv_lon;
         if (!upwind)
@@ -247,9 +267,9 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
         else
           val = kl * (u2 * (V0 + Vp) - bl * fabs(u2) * (Vp - V0) - u1 * (V0 + Vm) +
                       bl * fabs(u1) * (V0 - Vm) - (u2 - u1) * V0);
-        const CT vj = sv[N + i];
-        const CT v1 = vj * (CT)hc_0 + sv[i] * (CT)hc_m;
-        const CT v2 = vj * (CT)hc_0 + sv[2 * N + i] * (CT)hc_p;
+        const CT vj = sv[1][li];
+        const CT v1 = vj * (CT)hc_0 + sv[0][li] * (CT)hc_m;
+        const CT v2 = vj * (CT)hc_0 + sv[2][li] * (CT)hc_p;
         const CT Vjp = j + 1 <= M - 2 ? (CT)Vg[o + P + i] : (CT)0;
         const CT Vjm = j >= 1 ? (CT)Vg[o - P + i] : (CT)0;
         const CT kt = (CT)cj.k_vadv_lat;
@@ -261,13 +281,13 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
       }
       if (PASS != 2) {
         const CT f0 = (CT)cj.f, c0 = (CT)cj.c, fp = (CT)f_p, cp = (CT)c_p;
-        const CT a1 = (f0 + c0 * su[i]) * (CT)Ug[o + i];
-        const CT a2 = (f0 + c0 * su[im]) * (CT)Ug[o + im];
-        const CT a3 = (fp + cp * su[N + i]) * (CT)Ug[o + P + i];
-        const CT a4 = (fp + cp * su[N + im]) * (CT)Ug[o + P + im];
+        const CT a1 = (f0 + c0 * su[0][li]) * (CT)Ug[o + i];
+        const CT a2 = (f0 + c0 * su[0][li - 1]) * (CT)Ug[o + im];
+        const CT a3 = (fp + cp * su[1][li]) * (CT)Ug[o + P + i];
+        const CT a4 = (fp + cp * su[1][li - 1]) * (CT)Ug[o + P + im];
         fu = (CT)0.25 * (a1 + a2 + a3 + a4);
         const double dphi = gd[o + P + i] + ghs[o + P + i] - gd[o + i] - ghs[o + i];
-        vpgf = (CT)0.5 * ((CT)sG[N + i] + (CT)sG[2 * N + i]) / (CT)cj.hdlat * (CT)cj.hcos * (CT)dphi;
+        vpgf = (CT)0.5 * ((CT)sG[1][li] + (CT)sG[2][li]) / (CT)cj.hdlat * (CT)cj.hcos * (CT)dphi;
       }
       CT d;
       if (PASS == 0)
@@ -285,46 +305,27 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
       if (j == 0 || j == M - 1) {
         mdp = pole;
       } else {
-        const double Gi = sG[N + i], Gip = sG[N + ip], Gim = sG[N + im];
+        const double Gi = sG[1][li], Gip = sG[1][li + 1], Gim = sG[1][li - 1];
         mdl = ((Gi + Gip) * Ug[o + i] - (Gi + Gim) * Ug[o + im]) * 0.5 / cj.fdlon;
-        mdp = ((Gi + sG[2 * N + i]) * Vg[o + i] * hc_0 - (Gi + sG[i]) * Vg[o - P + i] * hc_m) * 0.5 / cj.fdlat;
+        mdp = ((Gi + sG[2][li]) * Vg[o + i] * hc_0 - (Gi + sG[0][li]) * Vg[o - P + i] * hc_m) * 0.5 / cj.fdlat;
       }
       dgd = -mdl - mdp;
     }
     // ---- filter staging or epilogue ----
-    if (st_du >= 0) {
+    if (st_du >= 0)
       sx[(size_t)st_du * P + i] = du;
-      s1_du += du * Ug[o + i];
-    } else {
+    else
       phase_epilogue<CT>(a, mb, CDU, j, i, du, cj.fcos, p1[0], p2[0]);
-    }
     if (has_v) {
-      if (st_dv >= 0) {
+      if (st_dv >= 0)
         sx[(size_t)st_dv * P + i] = dv;
-        s1_dv += dv * Vg[o + i];
-      } else {
+      else
         phase_epilogue<CT>(a, mb, CDV, j, i, dv, cj.hcos, p1[1], p2[1]);
-      }
     }
-    if (st_dgd >= 0) {
+    if (st_dgd >= 0)
       sx[(size_t)st_dgd * P + i] = dgd;
-      s1_dgd += dgd * (gd[o + i] + ghs[o + i]);
-    } else {
+    else
       phase_epilogue<CT>(a, mb, CDGD, j, i, dgd, cj.fcos, p1[2], p2[2]);
-    }
-  }
-  // row reductions (uniform control flow: every branch depends on j only)
-  if (st_du >= 0) {
-    double s = block_sum(s1_du, red);
-    if (tid == 0) mb.s1[st_du] = s;
-  }
-  if (st_dv >= 0) {
-    double s = block_sum(s1_dv, red);
-    if (tid == 0) mb.s1[st_dv] = s;
-  }
-  if (st_dgd >= 0) {
-    double s = block_sum(s1_dgd, red);
-    if (tid == 0) mb.s1[st_dgd] = s;
   }
   if (a.phase == PH_C) {
     for (int c = 0; c < 3; ++c) {
@@ -333,7 +334,7 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
       double s1v = block_sum(p1[c], red);
       double s2v = block_sum(p2[c], red);
       if (tid == 0) {
-        double *pp = mb.part + ((size_t)(j * 3 + c) * a.nchunk_row) * 2;
+        double *pp = mb.part + ((size_t)(j * 3 + c) * a.nchunk_row + i0 / kChunk) * 2;
         pp[0] = s1v;
         pp[1] = s2v;
       }
@@ -342,45 +343,65 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
 }
 
 // ---------------------------------------------------------------------------
-// corrector update (predict_correct, dycore_mod.F90:784-790)
+// beta of predict_correct (dycore_mod.F90:784-787), one workgroup per member
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_beta(const Args a) {
+  const int m = a.m0 + blockIdx.x;
+  const Member &mb = a.mem[m];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const int M = a.M, nc = a.nchunk_row;
+  // inner_product(tend(old), tend(new)) and (tend(new), tend(new)):
+  // du rows, then dv rows, then dgd rows (types_mod.F90:347-371)
+  double ip1 = 0, ip2 = 0;
+  const int per_comp = M * nc;
+  for (int c = 0; c < 3; ++c) {
+    double q1 = 0, q2 = 0;
+    for (int k0 = 0; k0 < per_comp; k0 += 4 * kBlock) {
+      double x1[4], x2[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + u * kBlock + tid;
+        x1[u] = x2[u] = 0.0;
+        if (k < per_comp) {
+          const int row = k / nc, ch = k - (k / nc) * nc;
+          const double *pp = mb.part + ((size_t)(row * 3 + c) * nc + ch) * 2;
+          x1[u] = pp[0];
+          x2[u] = pp[1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        q1 += x1[u];
+        q2 += x2[u];
+      }
+    }
+    ip1 += block_sum(q1, red);
+    ip2 += block_sum(q2, red);
+  }
+  if (tid == 0) mb.beta[0] = (a.qcon && ip1 != 0.0 && ip2 != 0.0) ? ip1 / ip2 : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// corrector update (predict_correct, dycore_mod.F90:789-790), pointwise
 // ---------------------------------------------------------------------------
 
 template <typename CT>
 __global__ void __launch_bounds__(kBlock) k_correct(const Args a) {
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
-  const int N = a.N, M = a.M, P = a.pitch;
-  const int m = a.m0 + t / M, j = t % M;
-  const Member &mb = a.mem[m];
-  __shared__ double red[16];
-  const int tid = threadIdx.x;
-  // inner_product(tend(old), tend(new)) and (tend(new), tend(new)):
-  // du rows, then dv rows, then dgd rows (types_mod.F90:347-371)
-  double ip1 = 0, ip2 = 0;
-  const int per_comp = M * a.nchunk_row;
-  for (int c = 0; c < 3; ++c) {
-    double q1 = 0, q2 = 0;
-    for (int k = tid; k < per_comp; k += kBlock) {
-      const int row = k / a.nchunk_row, ch = k % a.nchunk_row;
-      const double *pp = mb.part + ((size_t)(row * 3 + c) * a.nchunk_row + ch) * 2;
-      q1 += pp[0];
-      q2 += pp[1];
-    }
-    ip1 += block_sum(q1, red);
-    ip2 += block_sum(q2, red);
-  }
-  const double beta = (a.qcon && ip1 != 0.0 && ip2 != 0.0) ? ip1 / ip2 : 1.0;
-  const double dtb = a.dt * beta;
-  if (j == 0 && tid == 0) mb.beta[0] = beta;
-  const size_t o = (size_t)j * P;
-  const double *tgd = (const double *)mb.tc[CDGD];
-  const CT *tdu = (const CT *)mb.tc[CDU];
-  const CT *tdv = (const CT *)mb.tc[CDV];
-  for (int i = tid; i < N; i += kBlock) {
-    mb.st[a.s_out][FGD][o + i] = mb.st[a.s0][FGD][o + i] + dtb * tgd[o + i];
-    mb.st[a.s_out][FU][o + i] = mb.st[a.s0][FU][o + i] + dtb * (double)tdu[o + i];
-    if (j <= M - 2) mb.st[a.s_out][FV][o + i] = mb.st[a.s0][FV][o + i] + dtb * (double)tdv[o + i];
-  }
+  const int N = a.N, M = a.M, P = a.pitch, nseg = a.nseg;
+  const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
+  const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
+  const Member &mb = a.mem[a.m0 + mloc];
+  const int i = g * kBlock + threadIdx.x;
+  if (i >= N) return;
+  const double dtb = a.dt * mb.beta[0];
+  const size_t o = (size_t)j * P + i;
+  mb.st[a.s_out][FGD][o] = mb.st[a.s0][FGD][o] + dtb * ((const double *)mb.tc[CDGD])[o];
+  mb.st[a.s_out][FU][o] = mb.st[a.s0][FU][o] + dtb * (double)((const CT *)mb.tc[CDU])[o];
+  if (j <= M - 2) mb.st[a.s_out][FV][o] = mb.st[a.s0][FV][o] + dtb * (double)((const CT *)mb.tc[CDV])[o];
 }
 
 // ---------------------------------------------------------------------------
@@ -667,6 +688,7 @@ __device__ __forceinline__ void filter_coef_body(const Args &a, const FiltTile *
   int *kept = (int *)(rq + 16);               // [8]
   CT *xs = (CT *)(smem + 512);                // [16][BP]
   CT *bs = xs + 16 * BP;                      // [64][BP]
+  double *xd = (double *)(smem + 512 + (size_t)(16 + 64) * BP * 8);  // [16][64] FP64 copy (chunk 0)
   __shared__ double s2w[4][kFiltRows];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool want_w = a.phase <= PH_T;
@@ -693,25 +715,63 @@ __device__ __forceinline__ void filter_coef_body(const Args &a, const FiltTile *
   typename Mfma<CT>::acc_t acc = {0, 0, 0, 0};
   const int s_mine = slo + wave * 16 + (lane & 15);
   const bool s_ok = s_mine < hi;
+  // register double buffer: each thread owns 4 values of the 16 x 64 operand
+  // block (row tid/16, columns 4*(tid%16)..+3) and 16 values of the 64 x 64
+  // basis block (row tid/4, columns 16*(tid%4)..+15); the loads of block b+1
+  // are in flight while the MFMAs consume block b from LDS.
+  const int xr = tid >> 4, xc = (tid & 15) * 4;
+  const int br = tid >> 2, bc = (tid & 3) * 16;
+  __syncthreads();  // row pointers visible
+  const double *xp = rp[xr], *xq = rq[xr];
+  double xv[4];
+  CT bv[16];
+  auto load_block = [&](int l0) {
+    const int l = l0 + xc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double v = 0.0;
+      if (xp && l + k < N) v = xq ? xp[l + k] + xq[l + k] : xp[l + k];
+      xv[k] = v;
+    }
+    const int lb = l0 + br;
+    if (lb < N) {
+      const CT *src = Bt + (size_t)lb * a.spad + slo + bc;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) bv[k] = src[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) bv[k] = (CT)0;
+    }
+  };
+  // chunk 0 also forms s1 = <x, w> in FP64, the row inner product before
+  // filtering (dycore_mod.F90:213)
+  const bool do_s1 = chunk == 0 && want_w;
+  double s1p = 0.0;
+  load_block(0);
   for (int l0 = 0; l0 < N; l0 += LB) {
     __syncthreads();
-    for (int e = tid; e < 16 * LB; e += kBlock) {
-      const int r = e >> 6, l = l0 + (e & 63);
-      const double *p = rp[r], *q = rq[r];
-      double v = 0.0;
-      if (l < N && p) v = q ? p[l] + q[l] : p[l];
-      xs[r * BP + (e & 63)] = (CT)v;
-    }
-    for (int e = tid; e < LB * LB; e += kBlock) {
-      const int kk = e >> 6, c = e & 63, l = l0 + kk;
-      bs[kk * BP + c] = l < N ? Bt[(size_t)l * a.spad + slo + c] : (CT)0;
-    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) xs[xr * BP + xc + k] = (CT)xv[k];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) bs[br * BP + bc + k] = bv[k];
+    if (do_s1)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) xd[xr * 64 + xc + k] = xv[k];
     __syncthreads();
+    if (do_s1 && xr < 8)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s1p += xd[xr * 64 + xc + k] * xd[(xr + 8) * 64 + xc + k];
+    if (l0 + LB < N) load_block(l0 + LB);
 #pragma unroll 4
     for (int kk = 0; kk < LB; kk += 4) {
       const int k = kk + (lane >> 4);
       acc = Mfma<CT>::run(xs[(lane & 15) * BP + k], bs[k * BP + wave * 16 + (lane & 15)], acc);
     }
+  }
+  if (do_s1) {
+    // the 16 threads of row xr are 16 consecutive lanes of one wave
+    s1p = group16_sum(s1p);
+    if ((tid & 15) == 0 && xr < nrows) a.mem[ft->member[xr]].s1[ft->slot[xr]] = s1p;
   }
   // acc[r] = C[acc_row(q, r)][slot s_mine]; weight row = staged row + 8
   const int qd = lane >> 4;
@@ -792,13 +852,28 @@ __device__ __forceinline__ void filter_apply_body(const Args &a, const FiltTile 
   const int col0 = chunk * kChunk;
   const int col = col0 + wave * 16 + (lane & 15);
   typename Mfma<CT>::acc_t acc = {0, 0, 0, 0};
+  // register double buffer of the 64 x 64 basis block (row tid/4, columns
+  // 16*(tid%4)..+15), see k_filter_coef
+  const int br = tid >> 2, bc = (tid & 3) * 16;
+  CT bv[16];
+  auto load_block = [&](int s0) {
+    const int sidx = s0 + br;
+    if (sidx < ks) {
+      const CT *src = B + (size_t)(lo + sidx) * a.lpad + col0 + bc;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) bv[k] = src[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) bv[k] = (CT)0;
+    }
+  };
+  load_block(0);
   for (int s0 = 0; s0 < ks; s0 += LB) {
     __syncthreads();
-    for (int e = tid; e < LB * LB; e += kBlock) {
-      const int kk = e >> 6, c = e & 63, sidx = s0 + kk;
-      bs[kk * BP + c] = sidx < ks ? B[(size_t)(lo + sidx) * a.lpad + col0 + c] : (CT)0;
-    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) bs[br * BP + bc + k] = bv[k];
     __syncthreads();
+    if (s0 + LB < ks) load_block(s0 + LB);
     const int kend = ks - s0 < LB ? ((ks - s0 + 3) & ~3) : LB;
     for (int kk = 0; kk < kend; kk += 4) {
       const int k = kk + (lane >> 4);
@@ -912,9 +987,8 @@ static inline int row_grid(int ntiles) { return ((ntiles + kXcd - 1) / kXcd) * k
 
 template <typename CT>
 static hipError_t launch_tend_t(const Args &a, int pass, bool in_full, hipStream_t s) {
-  const size_t lds = tend_lds_bytes(a.N, sizeof(CT));
   dim3 g(row_grid(a.ntiles)), b(kBlock);
-#define GD_TEND(P, F) hipLaunchKernelGGL((k_tend<CT, P, F>), g, b, lds, s, a)
+#define GD_TEND(P, F) hipLaunchKernelGGL((k_tend<CT, P, F>), g, b, 0, s, a)
   if (pass == 0) { if (in_full) GD_TEND(0, true); else GD_TEND(0, false); }
   else if (pass == 1) { if (in_full) GD_TEND(1, true); else GD_TEND(1, false); }
   else { if (in_full) GD_TEND(2, true); else GD_TEND(2, false); }
@@ -924,6 +998,11 @@ static hipError_t launch_tend_t(const Args &a, int pass, bool in_full, hipStream
 
 hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStream_t s) {
   return prec == 0 ? launch_tend_t<double>(a, pass, in_full, s) : launch_tend_t<float>(a, pass, in_full, s);
+}
+
+hipError_t launch_beta(const Args &a, int nmem, hipStream_t s) {
+  hipLaunchKernelGGL(k_beta, dim3(nmem), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_correct(const Args &a, int prec, hipStream_t s) {
@@ -966,12 +1045,6 @@ hipError_t launch_divvor(const Args &a, hipStream_t s) {
 // default (large num_lon).
 hipError_t configure_kernels(int max_lds) {
   const void *fns[] = {
-      (const void *)k_tend<double, 0, true>, (const void *)k_tend<double, 0, false>,
-      (const void *)k_tend<double, 1, true>, (const void *)k_tend<double, 1, false>,
-      (const void *)k_tend<double, 2, true>, (const void *)k_tend<double, 2, false>,
-      (const void *)k_tend<float, 0, true>,  (const void *)k_tend<float, 0, false>,
-      (const void *)k_tend<float, 1, true>,  (const void *)k_tend<float, 1, false>,
-      (const void *)k_tend<float, 2, true>,  (const void *)k_tend<float, 2, false>,
       (const void *)k_diffuse<double>,       (const void *)k_diffuse<float>,
       (const void *)k_filter_coef<double>,   (const void *)k_filter_coef<float>,
       (const void *)k_filter_apply<double>,  (const void *)k_filter_apply<float>};

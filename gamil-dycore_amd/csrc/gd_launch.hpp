@@ -11,18 +11,16 @@ namespace gd {
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// k_tend: G (4 rows, double), u (2 rows) and v (3 rows) in the compute
-// type, plus 16 doubles of reduction scratch at the end.
-__host__ __device__ inline size_t tend_lds_bytes(int N, size_t cts) {
-  return align16((size_t)4 * N * 8 + (size_t)5 * N * cts) + 16 * sizeof(double);
-}
 // k_diffuse: G (4 rows), u (3), v (3), all double, + 16 doubles.
 __host__ __device__ inline size_t diffuse_lds_bytes(int N) {
   return (size_t)10 * N * 8 + 16 * sizeof(double);
 }
 // k_filter_coef: 512 bytes of row pointers, a 16 x 64 operand block and a
-// 64 x 64 basis block (pitch 65), sized for FP64 operands.
-__host__ __device__ inline size_t filt_coef_lds_bytes() { return 512 + (size_t)(16 + 64) * 65 * 8; }
+// 64 x 64 basis block (pitch 65), sized for FP64 operands, and an FP64 copy
+// of the operand block for the s1 inner product.
+__host__ __device__ inline size_t filt_coef_lds_bytes() {
+  return 512 + (size_t)(16 + 64) * 65 * 8 + (size_t)16 * 64 * 8;
+}
 // k_filter_apply: a 64 x 64 basis block (pitch 65) + the tile's 8 x kpad
 // coefficients, sized for FP64 operands.
 __host__ __device__ inline size_t filt_apply_lds_bytes(int kpad) {
@@ -31,6 +29,7 @@ __host__ __device__ inline size_t filt_apply_lds_bytes(int kpad) {
 
 // prec: 0 = FP64 plain, 1 = mixed (FP32 compute type)
 hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStream_t s);
+hipError_t launch_beta(const Args &a, int nmem, hipStream_t s);
 hipError_t launch_correct(const Args &a, int prec, hipStream_t s);
 hipError_t launch_diffuse(const Args &a, int prec, hipStream_t s);
 // mode 0: iap_transform of X after diffusion; 1: X <- slot s_in with u, v

@@ -1,0 +1,169 @@
+! gd_capi_mod -- iso_c_binding interface to the C ABI of include/gamil_dycore.h.
+! Every argument is a plain C type; 2-D fields are passed as the address of
+! element (1,1) of the reference's halo'd arrays x(-1:N+2,-1:M+2) with the
+! leading dimension ld = N+4.
+module gd_capi_mod
+
+  use iso_c_binding
+
+  implicit none
+
+  integer(c_int), parameter :: GD_OK = 0
+  integer(c_int), parameter :: GD_ERR_INVALID_ARG = 1
+  integer(c_int), parameter :: GD_ERR_HIP = 2
+  integer(c_int), parameter :: GD_ERR_NAN = 3
+  integer(c_int), parameter :: GD_ERR_NOT_SUPPORTED = 4
+
+  type, bind(C) :: gd_config
+    integer(c_int32_t) :: num_lon
+    integer(c_int32_t) :: num_lat
+    real(c_double) :: time_step_size
+    integer(c_int32_t) :: split_scheme
+    integer(c_int32_t) :: subcycles
+    integer(c_int32_t) :: qcon_modified
+    integer(c_int32_t) :: uv_adv_scheme
+    real(c_double) :: uv_adv_upwind_lon_beta
+    real(c_double) :: uv_adv_upwind_lat_beta
+    integer(c_int32_t) :: use_diffusion
+    integer(c_int32_t) :: diffusion_order
+    real(c_double) :: diffusion_coef
+    integer(c_int32_t) :: use_zonal_tend_filter
+    integer(c_int32_t) :: n_cutoff
+    type(c_ptr) :: cutoff
+    integer(c_int32_t) :: precision
+    integer(c_int32_t) :: device
+  end type gd_config
+
+  type, bind(C) :: gd_ic_params
+    integer(c_int64_t) :: seed
+    integer(c_int32_t) :: nmax
+    real(c_double) :: slope
+    real(c_double) :: rms_height
+    real(c_double) :: mean_depth
+    integer(c_int32_t) :: n_mountains
+    integer(c_int64_t) :: mountain_seed
+    integer(c_int64_t) :: pert_seed
+    integer(c_int32_t) :: pert_nmax
+    real(c_double) :: pert_rms
+  end type gd_ic_params
+
+  interface
+
+    integer(c_int) function gd_create(cfg, n_members, h) bind(C, name='gd_create')
+      import :: c_int, c_int32_t, c_ptr, gd_config
+      type(gd_config), intent(in) :: cfg
+      integer(c_int32_t), value :: n_members
+      type(c_ptr), intent(out) :: h
+    end function gd_create
+
+    integer(c_int) function gd_destroy(h) bind(C, name='gd_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function gd_destroy
+
+    type(c_ptr) function gd_last_error(h) bind(C, name='gd_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end function gd_last_error
+
+    integer(c_int) function gd_set_state(h, member, u, v, gd, ghs, ld) bind(C, name='gd_set_state')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: member
+      real(c_double), intent(in) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+    end function gd_set_state
+
+    integer(c_int) function gd_diag(h, total_mass, total_energy) bind(C, name='gd_diag')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: total_mass(*), total_energy(*)
+    end function gd_diag
+
+    integer(c_int) function gd_step(h, nsteps, total_mass, total_energy, beta, nan_step) &
+        bind(C, name='gd_step')
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsteps
+      real(c_double), intent(out) :: total_mass(*), total_energy(*), beta(*)
+      integer(c_int32_t), intent(out) :: nan_step
+    end function gd_step
+
+    integer(c_int) function gd_get_state(h, member, u, v, gd, ld) bind(C, name='gd_get_state')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: member
+      real(c_double), intent(out) :: u(*), v(*), gd(*)
+      integer(c_int64_t), value :: ld
+    end function gd_get_state
+
+    integer(c_int) function gd_get_diag_fields(h, member, div, vor, ld) &
+        bind(C, name='gd_get_diag_fields')
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: member
+      real(c_double), intent(out) :: div(*), vor(*)
+      integer(c_int64_t), value :: ld
+    end function gd_get_diag_fields
+
+    integer(c_int) function gd_ic_balanced(num_lon, num_lat, p, u, v, gd, ghs, ld) &
+        bind(C, name='gd_ic_balanced')
+      import :: c_int, c_int32_t, c_int64_t, c_double, gd_ic_params
+      integer(c_int32_t), value :: num_lon, num_lat
+      type(gd_ic_params), intent(in) :: p
+      real(c_double), intent(out) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+    end function gd_ic_balanced
+
+    integer(c_int) function gd_ic_rossby_haurwitz(num_lon, num_lat, R, omg, gd0, u, v, gd, ghs, ld) &
+        bind(C, name='gd_ic_rossby_haurwitz')
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: num_lon, num_lat
+      real(c_double), value :: R, omg, gd0
+      real(c_double), intent(out) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+    end function gd_ic_rossby_haurwitz
+
+    integer(c_int) function gd_ic_mountain_zonal_flow(num_lon, num_lat, u, v, gd, ghs, ld) &
+        bind(C, name='gd_ic_mountain_zonal_flow')
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: num_lon, num_lat
+      real(c_double), intent(out) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+    end function gd_ic_mountain_zonal_flow
+
+    integer(c_int) function gd_cutoff_from_lat0(num_lon, num_lat, lat0_deg, cutoff, cap) &
+        bind(C, name='gd_cutoff_from_lat0')
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: num_lon, num_lat
+      real(c_double), value :: lat0_deg
+      integer(c_int32_t), intent(out) :: cutoff(*)
+      integer(c_int32_t), value :: cap
+    end function gd_cutoff_from_lat0
+
+  end interface
+
+contains
+
+  ! Text of gd_last_error as a Fortran string.
+  function gd_error_text(h) result(res)
+
+    type(c_ptr), intent(in) :: h
+    character(512) :: res
+
+    type(c_ptr) p
+    character(kind=c_char), pointer :: s(:)
+    integer k
+
+    res = ''
+    p = gd_last_error(h)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [512])
+    do k = 1, 512
+      if (s(k) == c_null_char) exit
+      res(k:k) = s(k)
+    end do
+
+  end function gd_error_text
+
+end module gd_capi_mod

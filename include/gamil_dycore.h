@@ -184,6 +184,15 @@ int gd_ic_rossby_haurwitz(int32_t num_lon, int32_t num_lat, double R, double omg
 int gd_ic_mountain_zonal_flow(int32_t num_lon, int32_t num_lat, double *u, double *v,
                               double *gd, double *ghs, int64_t ld);
 
+/* Polar filter table whose kept wavenumber falls toward the poles as
+ * cos(lat)/cos(lat0): entry k (k = 1.. from each pole, the layout of
+ * zonal_tend_filter_cutoff_wavenumber, filter_mod.F90:44-99) drives full row
+ * 1+k; c_k = floor(num_lon/2 * cos(lat_{1+k}) / cos(lat0)) clipped to
+ * [1, num_lon/2-1], for every row poleward of lat0.  Writes at most cap
+ * entries and returns the number of entries (or -1 on bad arguments). */
+int gd_cutoff_from_lat0(int32_t num_lon, int32_t num_lat, double lat0_deg, int32_t *cutoff,
+                        int32_t cap);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* Runs nsteps steps launching every kernel eagerly with HIP events around
