@@ -20,7 +20,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("model-days/sec at 720x361 (mixed vs FP64 plain, same GPU); "
           "p50/p99 run latency")
 
-# The 720x361 workload; values not fixed by the workload
+# The 720x361 (0.5 degree) 10-day workload; values not fixed by the workload
 # description are stated here and in namelists/cfg2_720x361.nml.
 WORKLOAD = dict(num_lon=720, num_lat=361, time_step_size=120.0, split_scheme="csp2",
                 subcycles=6, qcon_modified=True, uv_adv_scheme="center_diff",
@@ -89,6 +89,26 @@ def run_gpu(gd, prec, ic, steps, warmup, repeats):
     return dict(elapsed=elapsed, mass=mass[:, 0], energy=energy[:, 0], lat=lat, prof=prof, cut=cut)
 
 
+def pmc_probe(gd, ic, nsteps):
+    """Workload of the hardware-counter runs: a plain device copy of 256 MiB
+    (its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the byte counters),
+    then nsteps steps of the mixed workload.  Run with GD_NO_GRAPH=1 so each
+    kernel is its own dispatch."""
+    import torch
+    n = 64 << 20                          # 64 Mi float32 = 256 MiB
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copy calibration: {n * 4} bytes read and written", flush=True)
+    d, _ = make_dycore(gd, "mixed")
+    d.set_state(*ic)
+    d.step(nsteps)
+    d.close()
+    print(f"pmc probe: {nsteps} steps done", flush=True)
+
+
 def cpu_baseline(ic, nsteps):
     """The FP64 C oracle (single thread, the reference is serial) on a bounded
     sample of the same workload: the first nsteps steps."""
@@ -121,15 +141,21 @@ def main():
     ap.add_argument("--repeats", type=int, default=3, help="full 10-day runs for p50/p99 latency")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", type=int, default=0, metavar="K",
+                    help="counter runs: one 256 MiB device copy (calibration of the byte "
+                         "counters) then K steps of the mixed workload, no timing output")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                                   "(sharding across GPUs is designed, not built)"}))
         sys.exit(2)
     import __graft_entry__ as ge
     gd = ge.load_package()
     N, M, dt = WORKLOAD["num_lon"], WORKLOAD["num_lat"], WORKLOAD["time_step_size"]
     ic = gd.ic_balanced(N, M, **IC)
+    if args.pmc_probe:
+        pmc_probe(gd, ic, args.pmc_probe)
+        return
     res = {}
     for prec in ("mixed", "fp64"):
         res[prec] = run_gpu(gd, prec, ic, args.steps, args.warmup, args.repeats)

@@ -57,7 +57,7 @@ EXPORTS = (
     "gd_get_state", "gd_get_diag_fields", "gd_space_operators", "gd_predict_correct",
     "gd_ordinary_diffusion", "gd_filter_rows", "gd_filter_table", "gd_ic_balanced",
     "gd_ic_rossby_haurwitz", "gd_ic_mountain_zonal_flow", "gd_profile_kernels",
-    "gd_device_bytes", "gd_selftest_mfma", "gd_cutoff_from_lat0",
+    "gd_device_bytes", "gd_cutoff_from_lat0",
 )
 
 _lib = None
@@ -98,8 +98,6 @@ def lib():
     L.gd_cutoff_from_lat0.argtypes = [C.c_int32, C.c_int32, C.c_double, ip, C.c_int32]
     L.gd_device_bytes.restype = C.c_int64
     L.gd_device_bytes.argtypes = [vp]
-    L.gd_selftest_mfma.argtypes = [dp, dp, dp, C.POINTER(C.c_float), C.POINTER(C.c_float),
-                                   C.POINTER(C.c_float)]
     _lib = L
     return L
 
@@ -267,12 +265,3 @@ class Dycore:
 
     def device_bytes(self):
         return int(lib().gd_device_bytes(self.h))
-
-
-def selftest_mfma(A, B, Af, Bf):
-    D = np.zeros((16, 16)); Df = np.zeros((16, 16), np.float32)
-    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
-    A = np.ascontiguousarray(A, np.float64); B = np.ascontiguousarray(B, np.float64)
-    Af = np.ascontiguousarray(Af, np.float32); Bf = np.ascontiguousarray(Bf, np.float32)
-    _check(lib().gd_selftest_mfma(_dp(A), _dp(B), _dp(D), fp(Af), fp(Bf), fp(Df)))
-    return D, Df

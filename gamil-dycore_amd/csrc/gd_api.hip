@@ -40,19 +40,16 @@ struct gd_handle {
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   int64_t bytes = 0;
-  std::vector<Member> hm;
-  Member *dm = nullptr;
+  char *arena = nullptr;         // all members, mstride bytes apart
+  size_t mstride = 0, fs = 0, cs = 0, off_part = 0, off_dpart = 0, off_sx = 0, off_beta = 0;
   RowCoef *dcoef = nullptr;
   std::vector<StageSlot> slots;
   StageSlot *dslots = nullptr;
-  std::vector<FiltTile> tiles;
-  FiltTile *dtiles = nullptr;
-  int kpad = 64, nchunk1 = 1, nchunk2 = 1;
-  double *dfcoef = nullptr, *dfs2 = nullptr;
-  void *dbasis = nullptr, *dbasis_t = nullptr;  // B[s][l] and Bt[l][s] in the compute type
-  void *dbasis64 = nullptr, *dbasis_t64 = nullptr;  // FP64 copies (aliases in FP64 mode)
-  int lpad = 64, spad = 128;
-  uint32_t *ddone = nullptr;
+  std::vector<FiltRow> frows;    // filter workgroups: (member, latitude) with staged rows
+  FiltRow *dfrows = nullptr;
+  double *dtwid = nullptr;       // [N][2] cos, sin of 2 pi k/N
+  int fft_L = 1, fft_half = 1, fft_nb = 1;
+  std::vector<int> fft_rad;
   int32_t *dstep = nullptr;
   double *ddiag = nullptr;  // [nmem][cap][3]
   int diag_cap = 512;
@@ -93,19 +90,24 @@ void *dalloc(gd_handle *h, size_t bytes, hipError_t &e) {
 
 Args base_args(const gd_handle *h) {
   Args a{};
-  a.mem = h->dm;
+  a.arena = h->arena;
+  a.mstride = h->mstride;
+  a.fs = h->fs;
+  a.cs = h->cs;
+  a.off_part = h->off_part;
+  a.off_dpart = h->off_dpart;
+  a.off_sx = h->off_sx;
+  a.off_beta = h->off_beta;
+  a.diag = h->ddiag;
   a.coef = h->dcoef;
   a.slots = h->dslots;
-  a.tiles = h->dtiles;
-  a.fcoef = h->dfcoef;
-  a.fs2 = h->dfs2;
-  a.basis = h->dbasis;
-  a.basis_t = h->dbasis_t;
-  a.basis64 = h->dbasis64;
-  a.basis_t64 = h->dbasis_t64;
-  a.lpad = h->lpad;
-  a.spad = h->spad;
-  a.done = h->ddone;
+  a.frows = h->dfrows;
+  a.twid = h->dtwid;
+  a.fft_L = h->fft_L;
+  a.fft_half = h->fft_half;
+  a.fft_nb = h->fft_nb;
+  a.fft_nrad = (int)h->fft_rad.size();
+  for (int k = 0; k < a.fft_nrad; ++k) a.fft_rad[k] = h->fft_rad[k];
   a.step_ctr = h->dstep;
   a.N = h->N;
   a.M = h->M;
@@ -113,13 +115,8 @@ Args base_args(const gd_handle *h) {
   a.nmem = h->nmem;
   a.m0 = 0;
   a.nseg = (h->N + kBlock - 1) / kBlock;
-  a.ntiles = h->nmem * h->M;
+  a.ntiles = h->nmem * h->M * a.nseg;  // (member, row, segment) tiles
   a.nslots = (int)h->slots.size();
-  a.nftiles = (int)h->tiles.size();
-  a.kpad = h->kpad;
-  a.nchunk1 = h->nchunk1;
-  a.nchunk2 = h->nchunk2;
-  a.nchunk_row = h->nchunk2;
   a.upwind = h->cfg.uv_adv_scheme == GD_ADV_UPWIND;
   a.beta_lon = h->cfg.uv_adv_upwind_lon_beta;
   a.beta_lat = h->cfg.uv_adv_upwind_lat_beta;
@@ -129,6 +126,8 @@ Args base_args(const gd_handle *h) {
   a.dlat = h->g.dlat;
   return a;
 }
+
+Member hmember(const gd_handle *h, int m) { return member_at(base_args(h), m); }
 
 // Runs one launch, bracketed by events when profiling.
 template <typename F>
@@ -146,12 +145,9 @@ hipError_t run(gd_handle *h, const char *family, F &&f) {
 }
 
 hipError_t enqueue_filter(gd_handle *h, Args a) {
-  if (h->tiles.empty()) return hipSuccess;
-  a.ntiles = (int)h->tiles.size() * h->nchunk1;
-  hipError_t e = run(h, "filter_coef", [&] { return launch_filter_coef(a, h->prec, h->stream); });
-  if (e != hipSuccess) return e;
-  a.ntiles = (int)h->tiles.size() * h->nchunk2;
-  return run(h, "filter_apply", [&] { return launch_filter_apply(a, h->prec, h->stream); });
+  if (h->frows.empty()) return hipSuccess;
+  a.ntiles = (int)h->frows.size();
+  return run(h, "filter", [&] { return launch_filter(a, h->prec, h->stream); });
 }
 
 // predict_correct(dt, s0 -> s_out, pass) (dycore_mod.F90:754-792)
@@ -202,8 +198,10 @@ hipError_t enqueue_finish(gd_handle *h, int mode, int s_in, int m0, int nm) {
   Args a = base_args(h);
   a.s_in = s_in;
   a.m0 = m0;
-  a.ntiles = nm * h->M;
-  return run(h, "finish", [&] { return launch_finish(a, mode, h->stream); });
+  a.ntiles = nm * h->M * a.nseg;
+  hipError_t e = run(h, "finish", [&] { return launch_finish(a, mode, h->stream); });
+  if (e != hipSuccess) return e;
+  return run(h, "diag", [&] { return launch_diag(a, nm, h->stream); });
 }
 
 // time_integrate (dycore_mod.F90:654-669) + diag_run; leaves the new state in X.
@@ -265,7 +263,7 @@ int build_step_graph(gd_handle *h) {
   return GD_OK;
 }
 
-// Staging slots, filter tiles and the per-row coefficient table.
+// Staging slots, filter workgroups and the per-row coefficient table.
 void build_tables(gd_handle *h, std::vector<RowCoef> &rc) {
   const Geometry &g = h->g;
   const int M = h->M, N = h->N;
@@ -306,57 +304,37 @@ void build_tables(gd_handle *h, std::vector<RowCoef> &rc) {
     if (g.half_flag[j]) add(CDV, j, g.half_cut[j]);
   for (int j = 0; j < M; ++j)
     if (g.full_flag[j]) add(CDGD, j, g.full_cut[j]);
-  // tiles: (member, slot) pairs sorted by kept count so a tile's rows share
-  // a slot range; ensemble members of one slot stay together.
-  // Tiles.  A row's arithmetic must not depend on its tile-mates, so that a
-  // run gives the same bits alone and batched: the projection mode is a
-  // property of the row (complement when more than half of the slots are
-  // kept), tiles hold rows of one mode, and slot ranges start on absolute
-  // 64-slot boundaries (kept mode at 0, complement mode at the smallest kept
-  // count rounded down), so every row sees the same k-grouping and chunking.
-  // Mass-continuity rows (dgd) form their own tiles: in mixed mode they run
-  // the FP64 GEMM so the filtered mass flux keeps FP64 accuracy.
-  struct Item { int cls, mode, kept, slot, member; };
-  std::vector<Item> items;
-  for (int s = 0; s < (int)h->slots.size(); ++s)
-    for (int m = 0; m < h->nmem; ++m) {
-      const int kept = h->slots[s].kept;
-      items.push_back({h->slots[s].comp == CDGD ? 1 : 0, 2 * kept > N ? 1 : 0, kept, s, m});
+  // filter workgroups: the staged rows of one latitude j of one member (du
+  // and dgd at full row j, dv at half row j).  A row's arithmetic does not
+  // depend on the other rows of its workgroup, so a member gives the same
+  // bits alone and inside an ensemble.
+  h->frows.clear();
+  for (int m = 0; m < h->nmem; ++m)
+    for (int j = 0; j < M; ++j) {
+      FiltRow f{};
+      f.member = m;
+      for (int c = 0; c < 3; ++c)
+        if (rc[j].stage[c] >= 0) {
+          const StageSlot &ss = h->slots[rc[j].stage[c]];
+          f.slot[f.nst] = rc[j].stage[c];
+          f.comp[f.nst] = ss.comp;
+          f.row[f.nst] = ss.row;
+          f.cut[f.nst] = (ss.kept - 2) / 2;
+          ++f.nst;
+        }
+      if (f.nst) h->frows.push_back(f);
     }
-  std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
-    if (x.cls != y.cls) return x.cls < y.cls;
-    if (x.mode != y.mode) return x.mode < y.mode;
-    return x.kept != y.kept ? x.kept < y.kept : (x.slot != y.slot ? x.slot < y.slot : x.member < y.member);
-  });
-  h->tiles.clear();
-  int kmax_range = 1;
-  for (size_t b = 0; b < items.size();) {
-    FiltTile t{};
-    size_t e = b;
-    while (e < items.size() && e - b < (size_t)kFiltRows && items[e].cls == items[b].cls &&
-           items[e].mode == items[b].mode)
-      ++e;
-    t.nrows = (int)(e - b);
-    t.fp64 = h->prec == 0 || items[b].cls == 1;
-    int kmin = N, kmax = 0;
-    for (int r = 0; r < t.nrows; ++r) {
-      t.member[r] = items[b + r].member;
-      t.slot[r] = items[b + r].slot;
-      kmin = std::min(kmin, items[b + r].kept);
-      kmax = std::max(kmax, items[b + r].kept);
-    }
-    if (items[b].mode == 0) {
-      t.complement = 0; t.lo = 0; t.hi = kmax;
-    } else {
-      t.complement = 1; t.lo = (kmin / kChunk) * kChunk; t.hi = N;
-    }
-    kmax_range = std::max(kmax_range, t.hi - t.lo);
-    h->tiles.push_back(t);
-    b = e;
-  }
-  h->kpad = ((kmax_range + kChunk - 1) / kChunk) * kChunk;
-  h->nchunk1 = h->kpad / kChunk;
-  h->nchunk2 = (N + kChunk - 1) / kChunk;
+}
+
+// Radix passes of the complex FFT of length L: 4s first, then 2, 3, 5, 7
+// and the remaining prime factors.
+std::vector<int> fft_plan(int L) {
+  std::vector<int> r;
+  while (L % 4 == 0) { r.push_back(4); L /= 4; }
+  while (L % 2 == 0) { r.push_back(2); L /= 2; }
+  for (int p = 3; L > 1; p += 2)
+    while (L % p == 0) { r.push_back(p); L /= p; }
+  return r;
 }
 
 std::string validate(const gd_config &c, int nmem) {
@@ -388,7 +366,7 @@ hipError_t d2h_rows(gd_handle *h, double *dst, const double *src, int rows, int6
 int read_tend(gd_handle *h, const Member &mb, int comp, double *out, int rows, int64_t ld) {
   if (!out) return GD_OK;
   std::vector<char> buf((size_t)rows * h->P * (comp == CDGD ? 8 : h->cts));
-  GD_HIP(h, hipMemcpyAsync(buf.data(), mb.tc[comp], buf.size(), hipMemcpyDeviceToHost, h->stream));
+  GD_HIP(h, hipMemcpyAsync(buf.data(), mb.tc(comp), buf.size(), hipMemcpyDeviceToHost, h->stream));
   GD_HIP(h, hipStreamSynchronize(h->stream));
   for (int j = 0; j < rows; ++j)
     for (int i = 0; i < h->N; ++i) {
@@ -455,12 +433,17 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   }
   std::vector<RowCoef> rc;
   build_tables(h, rc);
-  const size_t lds_need = std::max({diffuse_lds_bytes(h->N),
-                                    filt_coef_lds_bytes(), filt_apply_lds_bytes(h->kpad)});
-  if (lds_need > prop.sharedMemPerBlock) {
+  h->fft_half = h->N % 2 == 0;
+  h->fft_L = h->fft_half ? h->N / 2 : h->N;
+  h->fft_rad = fft_plan(h->fft_L);
+  h->fft_nb = 0;
+  for (int nb = kFiltMaxRows; nb >= 1 && !h->fft_nb; --nb)
+    if (filter_lds_bytes(h->N, h->fft_L, nb) <= prop.sharedMemPerBlock) h->fft_nb = nb;
+  if (!h->frows.empty() && (h->fft_nb == 0 || h->N > 16 * kBlock || (int)h->fft_rad.size() > kMaxRad)) {
     delete h;
-    return fail(nullptr, GD_ERR_INVALID_ARG, "num_lon too large for the LDS of this device");
+    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "num_lon too large for the polar filter (at most 4096)");
   }
+  if (!h->fft_nb) h->fft_nb = 1;
   configure_kernels((int)prop.sharedMemPerBlock);
   hipGetLastError();
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -472,98 +455,41 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   const size_t fld = (size_t)h->M * h->P;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t nslots = h->slots.size();
-  const size_t per_member = al(fld * 8) * (NSLOT * NFLD + 1 + 2 + 2) +  // states, ghs, tb/tc dgd, div/vor
-                            al(fld * h->cts) * 4 +                        // tb/tc du, dv
-                            al((size_t)h->M * 3 * h->nchunk2 * 2 * 8) + al((size_t)h->M * 4 * 8) +
-                            al(std::max<size_t>(nslots, 1) * h->P * 8) +
-                            al(std::max<size_t>(nslots, 1) * 8) + al(8);
-  h->hm.resize(h->nmem);
-  for (int m = 0; m < h->nmem; ++m) {
-    char *base = (char *)dalloc(h, per_member, e);
-    if (!base) { gd_destroy(h); return fail(nullptr, GD_ERR_HIP, "device allocation failed"); }
-    e = hipMemsetAsync(base, 0, per_member, h->stream);
-    Member &mb = h->hm[m];
-    char *p = base;
-    auto take = [&](size_t b) { char *q = p; p += al(b); return q; };
-    for (int s = 0; s < NSLOT; ++s)
-      for (int f = 0; f < NFLD; ++f) mb.st[s][f] = (double *)take(fld * 8);
-    mb.ghs = (double *)take(fld * 8);
-    mb.tb[CDU] = take(fld * h->cts); mb.tb[CDV] = take(fld * h->cts); mb.tb[CDGD] = take(fld * 8);
-    mb.tc[CDU] = take(fld * h->cts); mb.tc[CDV] = take(fld * h->cts); mb.tc[CDGD] = take(fld * 8);
-    mb.div = (double *)take(fld * 8);
-    mb.vor = (double *)take(fld * 8);
-    mb.part = (double *)take((size_t)h->M * 3 * h->nchunk2 * 2 * 8);
-    mb.dpart = (double *)take((size_t)h->M * 4 * 8);
-    mb.sx = (double *)take(std::max<size_t>(nslots, 1) * h->P * 8);
-    mb.s1 = (double *)take(std::max<size_t>(nslots, 1) * 8);
-    mb.beta = (double *)take(8);
-  }
+  const size_t a_nseg = (h->N + kBlock - 1) / kBlock;
+  h->fs = al(fld * 8);
+  h->cs = al(fld * h->cts);
+  h->off_part = (size_t)(NSLOT * NFLD + 5) * h->fs + 4 * h->cs;
+  h->off_dpart = h->off_part + al((size_t)h->M * 3 * a_nseg * 2 * 8);
+  h->off_sx = h->off_dpart + al((size_t)h->M * a_nseg * 4 * 8);
+  h->off_beta = h->off_sx + al(std::max<size_t>(nslots, 1) * h->P * 8);
+  h->mstride = h->off_beta + al(8);
+  h->arena = (char *)dalloc(h, h->mstride * h->nmem, e);
+  if (!h->arena) { gd_destroy(h); return fail(nullptr, GD_ERR_HIP, "device allocation failed"); }
+  e = hipMemsetAsync(h->arena, 0, h->mstride * h->nmem, h->stream);
   h->ddiag = (double *)dalloc(h, (size_t)h->nmem * h->diag_cap * 3 * 8, e);
-  for (int m = 0; m < h->nmem && h->ddiag; ++m) h->hm[m].diag_out = h->ddiag + (size_t)m * h->diag_cap * 3;
-  h->dm = (Member *)dalloc(h, sizeof(Member) * h->nmem, e);
   h->dcoef = (RowCoef *)dalloc(h, sizeof(RowCoef) * h->M, e);
   h->dslots = (StageSlot *)dalloc(h, sizeof(StageSlot) * std::max<size_t>(nslots, 1), e);
-  h->dtiles = (FiltTile *)dalloc(h, sizeof(FiltTile) * std::max<size_t>(h->tiles.size(), 1), e);
-  h->dfcoef = (double *)dalloc(h, std::max<size_t>(h->tiles.size(), 1) * kFiltRows * h->kpad * 8, e);
-  h->dfs2 = (double *)dalloc(h, std::max<size_t>(h->tiles.size(), 1) * h->nchunk1 * kFiltRows * 8, e);
-  h->lpad = ((h->N + 63) / 64) * 64;
-  h->spad = h->lpad + 64;
-  h->dbasis = dalloc(h, (size_t)h->N * h->lpad * h->cts, e);
-  h->dbasis_t = dalloc(h, (size_t)h->N * h->spad * h->cts, e);
-  if (h->cts == 8) {
-    h->dbasis64 = h->dbasis;
-    h->dbasis_t64 = h->dbasis_t;
-  } else {
-    h->dbasis64 = dalloc(h, (size_t)h->N * h->lpad * 8, e);
-    h->dbasis_t64 = dalloc(h, (size_t)h->N * h->spad * 8, e);
-  }
-  h->ddone = (uint32_t *)dalloc(h, sizeof(uint32_t) * h->nmem, e);
+  h->dfrows = (FiltRow *)dalloc(h, sizeof(FiltRow) * std::max<size_t>(h->frows.size(), 1), e);
+  h->dtwid = (double *)dalloc(h, (size_t)h->N * 2 * 8, e);
   h->dstep = (int32_t *)dalloc(h, sizeof(int32_t) * h->nmem, e);
-  if (!h->ddiag || !h->dm || !h->dcoef || !h->dslots || !h->dtiles || !h->dfcoef || !h->dfs2 ||
-      !h->dbasis || !h->dbasis_t || !h->dbasis64 || !h->dbasis_t64 || !h->ddone || !h->dstep) {
+  if (!h->ddiag || !h->dcoef || !h->dslots || !h->dfrows || !h->dtwid ||
+      !h->dstep) {
     gd_destroy(h);
     return fail(nullptr, GD_ERR_HIP, "device allocation failed");
   }
-  // FFTPACK5 real basis: slot 0 -> 1, slot 2k-1 -> cos(k l 2pi/N), slot 2k ->
-  // sin(k l 2pi/N) (for even N the last slot is cos(N/2 ...)); angles reduced
-  // mod N exactly before the FP64 cos/sin.
-  std::vector<double> tab(2 * h->N);
-  for (int q = 0; q < h->N; ++q) {
-    const double ang = 2.0 * kPi * (double)q / (double)h->N;
-    tab[q] = std::cos(ang);
-    tab[h->N + q] = std::sin(ang);
+  // twiddle table of the filter FFT: cos and sin of 2 pi k/N, evaluated in
+  // extended precision and rounded once to FP64
+  std::vector<double> tw((size_t)h->N * 2);
+  for (int k = 0; k < h->N; ++k) {
+    const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)h->N;
+    tw[2 * k] = (double)cosl(ang);
+    tw[2 * k + 1] = (double)sinl(ang);
   }
-  std::vector<double> bb((size_t)h->N * h->lpad, 0.0), bt((size_t)h->N * h->spad, 0.0);
-  for (int sl = 0; sl < h->N; ++sl)
-    for (int l = 0; l < h->N; ++l) {
-      double v;
-      if (sl == 0) {
-        v = 1.0;
-      } else {
-        const long k = (sl + 1) / 2, qq = (k * l) % h->N;
-        v = (sl & 1) ? tab[qq] : tab[h->N + qq];
-      }
-      bb[(size_t)sl * h->lpad + l] = v;
-      bt[(size_t)l * h->spad + sl] = v;
-    }
-  std::vector<float> bbf, btf;
-  if (h->cts == 4) {
-    bbf.assign(bb.begin(), bb.end());
-    btf.assign(bt.begin(), bt.end());
-  }
-  hipMemcpyAsync(h->dm, h->hm.data(), sizeof(Member) * h->nmem, hipMemcpyHostToDevice, h->stream);
   hipMemcpyAsync(h->dcoef, rc.data(), sizeof(RowCoef) * h->M, hipMemcpyHostToDevice, h->stream);
   if (nslots) hipMemcpyAsync(h->dslots, h->slots.data(), sizeof(StageSlot) * nslots, hipMemcpyHostToDevice, h->stream);
-  if (!h->tiles.empty())
-    hipMemcpyAsync(h->dtiles, h->tiles.data(), sizeof(FiltTile) * h->tiles.size(), hipMemcpyHostToDevice, h->stream);
-  hipMemcpyAsync(h->dbasis64, bb.data(), bb.size() * 8, hipMemcpyHostToDevice, h->stream);
-  hipMemcpyAsync(h->dbasis_t64, bt.data(), bt.size() * 8, hipMemcpyHostToDevice, h->stream);
-  if (h->cts == 4) {
-    hipMemcpyAsync(h->dbasis, bbf.data(), bbf.size() * 4, hipMemcpyHostToDevice, h->stream);
-    hipMemcpyAsync(h->dbasis_t, btf.data(), btf.size() * 4, hipMemcpyHostToDevice, h->stream);
-    hipStreamSynchronize(h->stream);
-  }
-  hipMemsetAsync(h->ddone, 0, sizeof(uint32_t) * h->nmem, h->stream);
+  if (!h->frows.empty())
+    hipMemcpyAsync(h->dfrows, h->frows.data(), sizeof(FiltRow) * h->frows.size(), hipMemcpyHostToDevice, h->stream);
+  hipMemcpyAsync(h->dtwid, tw.data(), tw.size() * 8, hipMemcpyHostToDevice, h->stream);
   hipMemsetAsync(h->dstep, 0, sizeof(int32_t) * h->nmem, h->stream);
   e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {
@@ -599,11 +525,11 @@ int gd_set_state(gd_handle *h, int32_t member, const double *u, const double *v,
     for (int i = 0; i < N; ++i)
       if (!(gdp[(size_t)j * ld + i] > 0.0)) return fail(h, GD_ERR_INVALID_ARG, "gd must be positive");
   hipSetDevice(h->device);
-  Member &mb = h->hm[member];
-  GD_HIP(h, h2d_rows(h, mb.st[SX][Fu], u, M, ld));
-  GD_HIP(h, h2d_rows(h, mb.st[SX][Fv], v, M - 1, ld));
-  GD_HIP(h, h2d_rows(h, mb.st[SX][FGD], gdp, M, ld));
-  GD_HIP(h, h2d_rows(h, mb.ghs, ghs, M, ld));
+  const Member mb = hmember(h, member);
+  GD_HIP(h, h2d_rows(h, mb.st(SX, Fu), u, M, ld));
+  GD_HIP(h, h2d_rows(h, mb.st(SX, Fv), v, M - 1, ld));
+  GD_HIP(h, h2d_rows(h, mb.st(SX, FGD), gdp, M, ld));
+  GD_HIP(h, h2d_rows(h, mb.ghs(), ghs, M, ld));
   // iap_transform(state(old)) + diag (dycore_mod.F90:123-125)
   GD_HIP(h, enqueue_finish(h, 0, SX, member, 1));
   GD_HIP(h, hipStreamSynchronize(h->stream));
@@ -667,10 +593,10 @@ int gd_get_state(gd_handle *h, int32_t member, double *u, double *v, double *gdp
   if (rc) return rc;
   if (ld < h->N) return fail(h, GD_ERR_INVALID_ARG, "ld < num_lon");
   hipSetDevice(h->device);
-  const Member &mb = h->hm[member];
-  if (u) GD_HIP(h, d2h_rows(h, u, mb.st[SX][Fu], h->M, ld));
-  if (v) GD_HIP(h, d2h_rows(h, v, mb.st[SX][Fv], h->M - 1, ld));
-  if (gdp) GD_HIP(h, d2h_rows(h, gdp, mb.st[SX][FGD], h->M, ld));
+  const Member mb = hmember(h, member);
+  if (u) GD_HIP(h, d2h_rows(h, u, mb.st(SX, Fu), h->M, ld));
+  if (v) GD_HIP(h, d2h_rows(h, v, mb.st(SX, Fv), h->M - 1, ld));
+  if (gdp) GD_HIP(h, d2h_rows(h, gdp, mb.st(SX, FGD), h->M, ld));
   GD_HIP(h, hipStreamSynchronize(h->stream));
   return GD_OK;
 }
@@ -684,9 +610,9 @@ int gd_get_diag_fields(gd_handle *h, int32_t member, double *div, double *vor, i
   a.m0 = member;
   a.ntiles = h->M;
   GD_HIP(h, launch_divvor(a, h->stream));
-  const Member &mb = h->hm[member];
-  if (div) GD_HIP(h, d2h_rows(h, div, mb.div, h->M, ld));
-  if (vor) GD_HIP(h, d2h_rows(h, vor, mb.vor, h->M - 1, ld));
+  const Member mb = hmember(h, member);
+  if (div) GD_HIP(h, d2h_rows(h, div, mb.div(), h->M, ld));
+  if (vor) GD_HIP(h, d2h_rows(h, vor, mb.vor(), h->M - 1, ld));
   GD_HIP(h, hipStreamSynchronize(h->stream));
   return GD_OK;
 }
@@ -705,7 +631,7 @@ int gd_space_operators(gd_handle *h, int32_t member, int32_t pass, double *du, d
   a.ntiles = h->nmem * h->M * a.nseg;
   GD_HIP(h, launch_tend(a, h->prec, pass, true, h->stream));
   GD_HIP(h, enqueue_filter(h, a));
-  const Member &mb = h->hm[member];
+  const Member mb = hmember(h, member);
   if ((rc = read_tend(h, mb, CDU, du, h->M, ld))) return rc;
   if ((rc = read_tend(h, mb, CDV, dv, h->M - 1, ld))) return rc;
   if ((rc = read_tend(h, mb, CDGD, dgd, h->M, ld))) return rc;
@@ -719,7 +645,7 @@ int gd_predict_correct(gd_handle *h, double dt, int32_t pass, int32_t qcon, doub
   GD_HIP(h, enqueue_finish(h, 1, SY, 0, h->nmem));
   if (beta)
     for (int m = 0; m < h->nmem; ++m)
-      GD_HIP(h, hipMemcpyAsync(beta + m, h->hm[m].beta, 8, hipMemcpyDeviceToHost, h->stream));
+      GD_HIP(h, hipMemcpyAsync(beta + m, hmember(h, m).beta(), 8, hipMemcpyDeviceToHost, h->stream));
   GD_HIP(h, hipStreamSynchronize(h->stream));
   return GD_OK;
 }
@@ -731,7 +657,7 @@ int gd_ordinary_diffusion(gd_handle *h, double dt) {
   const size_t fb = (size_t)h->M * h->P * 8;
   for (int m = 0; m < h->nmem; ++m)
     for (int f = 0; f < NFLD; ++f)
-      GD_HIP(h, hipMemcpyAsync(h->hm[m].st[SY][f], h->hm[m].st[SX][f], fb, hipMemcpyDeviceToDevice, h->stream));
+      GD_HIP(h, hipMemcpyAsync(hmember(h, m).st(SY, f), hmember(h, m).st(SX, f), fb, hipMemcpyDeviceToDevice, h->stream));
   GD_HIP(h, enqueue_diffusion(h, SY, true, dt));
   GD_HIP(h, enqueue_finish(h, 0, SX, 0, h->nmem));
   GD_HIP(h, hipStreamSynchronize(h->stream));
@@ -742,11 +668,11 @@ int gd_filter_rows(gd_handle *h, int32_t half, double *x, int64_t ld) {
   if (!h || !x || ld < h->N) return GD_ERR_INVALID_ARG;
   hipSetDevice(h->device);
   const int comp = half ? CDV : CDGD;
-  const Member &mb = h->hm[0];
+  const Member mb = hmember(h, 0);
   const size_t rowb = (size_t)h->P * 8;
   for (size_t s = 0; s < h->slots.size(); ++s) {
     if (h->slots[s].comp != comp) continue;
-    GD_HIP(h, hipMemcpy(mb.sx + s * h->P, x + (size_t)h->slots[s].row * ld, (size_t)h->N * 8,
+    GD_HIP(h, hipMemcpy(mb.sx() + s * h->P, x + (size_t)h->slots[s].row * ld, (size_t)h->N * 8,
                         hipMemcpyHostToDevice));
   }
   Args a = base_args(h);
@@ -755,7 +681,7 @@ int gd_filter_rows(gd_handle *h, int32_t half, double *x, int64_t ld) {
   GD_HIP(h, hipStreamSynchronize(h->stream));
   for (size_t s = 0; s < h->slots.size(); ++s) {
     if (h->slots[s].comp != comp) continue;
-    GD_HIP(h, hipMemcpy(x + (size_t)h->slots[s].row * ld, mb.sx + s * h->P, (size_t)h->N * 8,
+    GD_HIP(h, hipMemcpy(x + (size_t)h->slots[s].row * ld, mb.sx() + s * h->P, (size_t)h->N * 8,
                         hipMemcpyDeviceToHost));
   }
   (void)rowb;
@@ -813,26 +739,5 @@ int gd_profile_kernels(gd_handle *h, int32_t nsteps, int32_t max_families, int32
 }
 
 int64_t gd_device_bytes(const gd_handle *h) { return h ? h->bytes : 0; }
-
-// Device check of the MFMA operand layout the filter kernels rely on:
-// D = A B with A 16x4 and B 4x16 (row-major), FP64 and FP32 variants.
-int gd_selftest_mfma(const double *A, const double *B, double *D, const float *Af, const float *Bf,
-                     float *Df) {
-  double *dA, *dB, *dD;
-  float *fA, *fB, *fD;
-  if (hipMalloc(&dA, 64 * 8) || hipMalloc(&dB, 64 * 8) || hipMalloc(&dD, 256 * 8) ||
-      hipMalloc(&fA, 64 * 4) || hipMalloc(&fB, 64 * 4) || hipMalloc(&fD, 256 * 4))
-    return GD_ERR_HIP;
-  hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice);
-  hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice);
-  hipMemcpy(fA, Af, 64 * 4, hipMemcpyHostToDevice);
-  hipMemcpy(fB, Bf, 64 * 4, hipMemcpyHostToDevice);
-  hipError_t e = launch_mfma_layout(dA, dB, dD, fA, fB, fD, nullptr);
-  if (e == hipSuccess) e = hipDeviceSynchronize();
-  hipMemcpy(D, dD, 256 * 8, hipMemcpyDeviceToHost);
-  hipMemcpy(Df, fD, 256 * 4, hipMemcpyDeviceToHost);
-  hipFree(dA); hipFree(dB); hipFree(dD); hipFree(fA); hipFree(fB); hipFree(fD);
-  return e == hipSuccess ? GD_OK : GD_ERR_HIP;
-}
 
 }  // extern "C"

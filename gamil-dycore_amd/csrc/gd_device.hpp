@@ -26,8 +26,8 @@ enum Fld { FGD = 0, FU = 1, FV = 2, Fu = 3, Fv = 4, NFLD = 5 };
 enum Comp { CDU = 0, CDV = 1, CDGD = 2 };
 enum Phase { PH_A = 0, PH_B = 1, PH_C = 2, PH_T = 3, PH_D = 4, PH_P = 5 };
 
-constexpr int kFiltRows = 8;     // staged rows per filter tile (+8 weight rows)
-constexpr int kChunk = 64;       // slots (F1) / columns (F2) per filter workgroup
+constexpr int kMaxRad = 24;      // radix passes of the filter FFT plan
+constexpr int kFiltMaxRows = 3;  // staged rows of one latitude (du, dv, dgd)
 
 // Per full row j (0-based).  Half-row entries at index j describe half row j
 // (0 for j = M-1).
@@ -43,18 +43,33 @@ struct RowCoef {
   int32_t pad;
 };
 
+// Per-member arena.  Members sit `mstride` bytes apart in one allocation and
+// every buffer is at a fixed offset from the member's base, so kernels form
+// pointers arithmetically from kernel arguments (no table in HBM to load):
+//   k * fs, k = slot * NFLD + field   state fields (double)
+//   25 fs ghs, 26 fs Tb dgd, 27 fs Tc dgd, 28 fs div, 29 fs vor (double)
+//   30 fs + q cs                      Tb du, Tb dv, Tc du, Tc dv (compute type)
+//   off_part, off_dpart, off_sx, off_beta  beta partials [M][3][nseg][2], diag
+//                                     partials [M][nseg][4], staged filter rows
+//                                     [nslots][pitch], beta of the last call
 struct Member {
-  double *st[NSLOT][NFLD];   // state fields (u/v read only from slot X on the step path)
-  double *ghs;
-  void *tb[3];               // Tb: du, dv (compute type), dgd (double)
-  void *tc[3];               // Tc
-  double *part;              // beta partials [M][3 comps][nchunk][2]
-  double *dpart;             // diag partials [M][4]
-  double *sx;                // staged raw filter rows [nslots][pitch]
-  double *s1;                // [nslots] row inner product before filtering
-  double *beta;              // [1] beta of the last integrator call
-  double *div, *vor;         // history diagnostics
-  double *diag_out;          // [cap][3]: mass, energy, beta per step
+  char *b;
+  size_t fs, cs, off_part, off_dpart, off_sx, off_beta;
+  double *diag_out;  // [cap][3]: mass, energy, beta per step
+  __host__ __device__ double *st(int slot, int f) const { return (double *)(b + (size_t)(slot * NFLD + f) * fs); }
+  __host__ __device__ double *ghs() const { return (double *)(b + (size_t)(NSLOT * NFLD) * fs); }
+  __host__ __device__ void *tb(int c) const {
+    return c == CDGD ? (void *)(b + (size_t)(NSLOT * NFLD + 1) * fs) : (void *)(b + (size_t)(NSLOT * NFLD + 5) * fs + (size_t)c * cs);
+  }
+  __host__ __device__ void *tc(int c) const {
+    return c == CDGD ? (void *)(b + (size_t)(NSLOT * NFLD + 2) * fs) : (void *)(b + (size_t)(NSLOT * NFLD + 5) * fs + (size_t)(2 + c) * cs);
+  }
+  __host__ __device__ double *div() const { return (double *)(b + (size_t)(NSLOT * NFLD + 3) * fs); }
+  __host__ __device__ double *vor() const { return (double *)(b + (size_t)(NSLOT * NFLD + 4) * fs); }
+  __host__ __device__ double *part() const { return (double *)(b + off_part); }
+  __host__ __device__ double *dpart() const { return (double *)(b + off_dpart); }
+  __host__ __device__ double *sx() const { return (double *)(b + off_sx); }
+  __host__ __device__ double *beta() const { return (double *)(b + off_beta); }
 };
 
 struct StageSlot {
@@ -64,44 +79,39 @@ struct StageSlot {
   int32_t pad;
 };
 
-// A filter tile: up to kFiltRows staged rows (member, slot) with one slot
-// range.  Kept mode: projection onto slots [0, hi).  Complement mode:
-// x minus the projection onto slots [lo, N).
-struct FiltTile {
-  int32_t nrows;
-  int32_t complement;
-  int32_t lo, hi;              // slot range handled by the GEMM
-  int32_t fp64;                // FP64 GEMM (all tiles in FP64 mode; dgd tiles in mixed mode)
-  int32_t pad;
-  int32_t member[kFiltRows];
-  int32_t slot[kFiltRows];
+// The staged rows of one latitude j of one member, filtered by one
+// workgroup: du at full row j, dv at half row j, dgd at full row j (those
+// that are flagged), with what the kernel needs of each row.
+struct FiltRow {
+  int32_t member;
+  int32_t nst;                     // staged rows (1..3)
+  int32_t slot[kFiltMaxRows];      // staging slots
+  int32_t comp[kFiltMaxRows];      // CDU / CDV / CDGD
+  int32_t row[kFiltMaxRows];       // full row (du, dgd) or half row (dv)
+  int32_t cut[kFiltMaxRows];       // cutoff wavenumber c
 };
 
-struct Ctrl {
-  int32_t step;                // diag ring write position (all members)
-  uint32_t done[1];            // per-member block counters follow (flexible)
-};
 
 struct Args {
-  const Member *mem;
+  char *arena;                 // member m at arena + m * mstride
+  size_t mstride, fs, cs, off_part, off_dpart, off_sx, off_beta;
+  double *diag;                // [nmem][diag_cap][3]
   const RowCoef *coef;
   const StageSlot *slots;
-  const FiltTile *tiles;
-  double *fcoef;               // filter coefficients [ntile][kFiltRows][kpad]
-  double *fs2;                 // s2 partials [ntile][nchunk1][kFiltRows]
-  const void *basis;           // B[s][l], slot-major, pitch lpad (compute type)
-  const void *basis_t;         // Bt[l][s], longitude-major, pitch spad
-  const void *basis64;         // FP64 copies (same as basis/basis_t in FP64 mode)
-  const void *basis_t64;
-  int lpad, spad;
-  uint32_t *done;              // per-member finish counters
+  const FiltRow *frows;
+  const double *twid;          // [N][2]: cos, sin of 2 pi k / N
+  int fft_L;                   // complex FFT length: N/2 (even N) or N
+  int fft_half;                // 1: even N, real FFT through a half-length complex FFT
+  int fft_nrad;                // radix passes
+  int fft_nb;                  // rows transformed together (LDS budget)
+  int fft_rad[kMaxRad];
   int32_t *step_ctr;
   int N, M, pitch, nmem;
   int m0;                      // first member of a row launch
   int nseg;                    // longitude segments of kBlock points per row
   int ntiles;                  // work items of this launch
   int in_full;                 // k_diffuse: read stored u, v of s_in
-  int nslots, nftiles, kpad, nchunk1, nchunk2, nchunk_row;
+  int nslots;
   int pass, phase;
   int s_in, s0, s_out;
   int upwind;
@@ -113,5 +123,18 @@ struct Args {
   double beta_lon, beta_lat;
   double dlon, dlat;
 };
+
+__host__ __device__ inline Member member_at(const Args &a, int m) {
+  Member mb;
+  mb.b = a.arena + (size_t)m * a.mstride;
+  mb.fs = a.fs;
+  mb.cs = a.cs;
+  mb.off_part = a.off_part;
+  mb.off_dpart = a.off_dpart;
+  mb.off_sx = a.off_sx;
+  mb.off_beta = a.off_beta;
+  mb.diag_out = a.diag + (size_t)m * a.diag_cap * 3;
+  return mb;
+}
 
 }  // namespace gd

@@ -1,0 +1,405 @@
+// gd_filter.hip -- the polar zonal filter (filter_mod.F90:105-172) fused with
+// the rescale of space_operators (dycore_mod.F90:210-217) and the phase
+// epilogue, one workgroup per (member, latitude) whose tendency rows are
+// staged.
+//
+// FFTPACK5's rfft1f -> mask -> rfft1b of one row is the orthogonal projection
+// onto the Fourier modes the mask keeps: wavenumbers 0..c and the cosine part
+// of wavenumber c+1 (the mask keeps array slots 1..2c+2, and slot 2c+2 is
+// a_{c+1} in FFTPACK5's [a0, a1, b1, a2, b2, ...] layout).  The kernel computes
+// it with a complex FFT in LDS:
+//   even N: the real row is read as N/2 complex points z_m = x_2m + i x_2m+1,
+//           transformed, split into the real spectrum X_0..X_{N/2}, masked,
+//           recombined and transformed back (the standard half-length real
+//           FFT);
+//   odd N:  the row is transformed as N complex points with zero imaginary
+//           part, masked symmetrically and transformed back.
+// The complex FFT is a Stockham autosort FFT (out-of-place passes between two
+// LDS buffers, so every pass is one read, one write and one barrier) with
+// radix-4/2 passes, radix-3/5/7 passes, and a direct O(R^2) pass for other
+// prime factors.  Twiddles W_N^k come from an FP64 table (cos, sin of
+// 2 pi k/N) loaded into LDS.  Everything is FP64 in every precision mode.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gd_common.hpp"
+#include "gd_device.hpp"
+#include "gd_launch.hpp"
+
+namespace gd {
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// W_N^m = exp(-2 pi i m/N) (forward) or its conjugate (inverse).
+__device__ __forceinline__ double2 twf(const double2 *tw, int m, bool inv) {
+  const double2 c = tw[m];
+  return make_double2(c.x, inv ? c.y : -c.y);
+}
+
+// In-register DFT of R points; W_R = W_N^rstep.
+template <int R>
+__device__ __forceinline__ void dft_small(double2 (&v)[R], const double2 *tw, int rstep, bool inv) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const double2 t2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+    // forward: -i (v1 - v3); inverse: +i (v1 - v3)
+    const double2 t3 = inv ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);
+    v[0] = cadd(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[2] = csub(t0, t2);
+    v[3] = csub(t1, t3);
+  } else {
+    double2 o[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      double2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) acc = cadd(acc, cmul(v[r], twf(tw, ((r * q) % R) * rstep, inv)));
+      o[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = o[q];
+  }
+}
+
+// One Stockham pass of radix R over nb rows of L points: butterfly j of a
+// row reads points j + q L/R, twiddles them by W_{Ns R}^{q (j mod Ns)} and
+// writes its outputs to ((j - j mod Ns) R + j mod Ns) + q Ns.
+template <int R>
+__device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L, int Ns, int nb,
+                                         const double2 *tw, int tstride, bool inv) {
+  const int Lr = L / R;
+  const int tws = (L / (Ns * R)) * tstride;
+  const int rstep = Lr * tstride;
+  for (int b = threadIdx.x; b < nb * Lr; b += kBlock) {
+    const int r = b / Lr, j = b - r * Lr;
+    const int k = j % Ns;
+    const double2 *s = src + (size_t)r * L;
+    double2 *d = dst + (size_t)r * L;
+    double2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = s[j + q * Lr];
+    if (Ns > 1) {
+#pragma unroll
+      for (int q = 1; q < R; ++q) v[q] = cmul(v[q], twf(tw, q * k * tws, inv));
+    }
+    dft_small<R>(v, tw, rstep, inv);
+    const int o = (j - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) d[o + q * Ns] = v[q];
+  }
+}
+
+// The same pass for any radix R (prime factors above 7), O(R^2) per butterfly.
+__device__ __forceinline__ void fft_pass_any(const double2 *src, double2 *dst, int L, int Ns, int R, int nb,
+                                             const double2 *tw, int tstride, bool inv) {
+  const int Lr = L / R;
+  const int tws = (L / (Ns * R)) * tstride;
+  const int rstep = Lr * tstride;
+  for (int b = threadIdx.x; b < nb * Lr; b += kBlock) {
+    const int r = b / Lr, j = b - r * Lr;
+    const int k = j % Ns;
+    const double2 *s = src + (size_t)r * L;
+    double2 *d = dst + (size_t)r * L;
+    const int o = (j - k) * R + k;
+    for (int q = 0; q < R; ++q) {
+      double2 acc = s[j];
+      for (int p = 1; p < R; ++p) {
+        double2 v = s[j + p * Lr];
+        if (Ns > 1) v = cmul(v, twf(tw, p * k * tws, inv));
+        acc = cadd(acc, cmul(v, twf(tw, ((p * q) % R) * rstep, inv)));
+      }
+      d[o + q * Ns] = acc;
+    }
+  }
+}
+
+// Runs the plan's passes starting from buffer `cur`; returns the buffer that
+// holds the result.  Ends with a barrier.
+__device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const Args &a, int nb,
+                                       const double2 *tw, bool inv) {
+  const int L = a.fft_L, tstride = a.fft_half ? 2 : 1;
+  int Ns = 1;
+  for (int p = 0; p < a.fft_nrad; ++p) {
+    const int R = a.fft_rad[p];
+    const double2 *src = cur ? b1 : b0;
+    double2 *dst = cur ? b0 : b1;
+    switch (R) {
+      case 2: fft_pass<2>(src, dst, L, Ns, nb, tw, tstride, inv); break;
+      case 3: fft_pass<3>(src, dst, L, Ns, nb, tw, tstride, inv); break;
+      case 4: fft_pass<4>(src, dst, L, Ns, nb, tw, tstride, inv); break;
+      case 5: fft_pass<5>(src, dst, L, Ns, nb, tw, tstride, inv); break;
+      case 7: fft_pass<7>(src, dst, L, Ns, nb, tw, tstride, inv); break;
+      default: fft_pass_any(src, dst, L, Ns, R, nb, tw, tstride, inv); break;
+    }
+    __syncthreads();
+    Ns *= R;
+    cur ^= 1;
+  }
+  return cur;
+}
+
+// FFTPACK5 mask of cutoff c on the complex spectrum: wavenumbers 0..c kept,
+// only the cosine (real) part of wavenumber c+1 kept, the rest removed.
+__device__ __forceinline__ double2 fmask(double2 X, int k, int c) {
+  if (k <= c) return X;
+  if (k == c + 1) return make_double2(X.x, 0.0);
+  return make_double2(0.0, 0.0);
+}
+
+// Even N: Z = FFT_L(z) of the packed row (L = N/2) -> real spectrum X_k =
+// E_k + W_N^k O_k with E_k = (Z_k + conj Z_{L-k})/2, O_k = -i (Z_k - conj
+// Z_{L-k})/2 -> mask -> the packed spectrum of the filtered row, Z'_k =
+// E'_k + i O'_k with E'_k = (Y_k + conj Y_{L-k})/2, O'_k = (Y_k - conj
+// Y_{L-k}) W_N^{-k}/2, scaled by 1/L for the unnormalised inverse.  Pairs
+// (k, L-k) are handled together; k = 0 carries X_0 and the Nyquist X_L.
+__device__ __forceinline__ void real_mid(const double2 *src, double2 *dst, int L, int nb, const int *cut,
+                                         const double2 *tw, double sc) {
+  const int H = L / 2 + 1;
+  for (int b = threadIdx.x; b < nb * H; b += kBlock) {
+    const int r = b / H, k = b - r * H;
+    const int c = cut[r];
+    const double2 *s = src + (size_t)r * L;
+    double2 *d = dst + (size_t)r * L;
+    if (k == 0) {
+      const double2 z = s[0];
+      const double X0 = z.x + z.y, XL = z.x - z.y;
+      const double YL = L <= c + 1 ? XL : 0.0;
+      d[0] = make_double2(0.5 * (X0 + YL) * sc, 0.5 * (X0 - YL) * sc);
+      continue;
+    }
+    const int km = L - k;
+    const double2 zk = s[k], zm = s[km];
+    const double2 E = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+    const double2 O = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+    const double2 wk = twf(tw, k, false);
+    const double2 wO = cmul(wk, O);
+    const double2 Xk = cadd(E, wO);
+    const double2 Xm = make_double2(E.x - wO.x, -(E.y - wO.y));  // X_{L-k} = conj(E - W^k O)
+    const double2 Yk = fmask(Xk, k, c), Ym = fmask(Xm, km, c);
+    const double2 Ep = make_double2(0.5 * (Yk.x + Ym.x), 0.5 * (Yk.y - Ym.y));
+    const double2 Dh = make_double2(0.5 * (Yk.x - Ym.x), 0.5 * (Yk.y + Ym.y));
+    const double2 Op = cmul(Dh, make_double2(wk.x, -wk.y));
+    d[k] = make_double2((Ep.x - Op.y) * sc, (Ep.y + Op.x) * sc);
+    if (km != k) d[km] = make_double2((Ep.x + Op.y) * sc, (Op.x - Ep.y) * sc);
+  }
+}
+
+// Odd N: mask the full complex spectrum symmetrically, scaled by 1/N.
+__device__ __forceinline__ void cplx_mid(const double2 *src, double2 *dst, int L, int nb, const int *cut,
+                                         double sc) {
+  for (int b = threadIdx.x; b < nb * L; b += kBlock) {
+    const int r = b / L, k = b - r * L;
+    const int kk = k <= L - k ? k : L - k;
+    const double2 Y = fmask(src[(size_t)r * L + k], kk, cut[r]);
+    dst[(size_t)r * L + k] = make_double2(Y.x * sc, Y.y * sc);
+  }
+}
+
+// Element i of a 3-array without dynamic indexing (which would go to scratch).
+__device__ __forceinline__ int pick3(const int32_t (&v)[kFiltMaxRows], int i) {
+  return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+
+// KP: longitudes per thread (N <= KP * kBlock).
+template <typename CT, int KP>
+__global__ void __launch_bounds__(kBlock) k_filter(const Args a) {
+  const int gi = blockIdx.x;
+  if (gi >= a.ntiles) return;
+  const FiltRow fr = a.frows[gi];
+  const Member mb = member_at(a, fr.member);
+  const int N = a.N, P = a.pitch, L = a.fft_L, tid = threadIdx.x;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2 *tw = (double2 *)smem;
+  double2 *b0 = tw + N;
+  double2 *b1 = b0 + (size_t)a.fft_nb * L;
+  __shared__ double red[4 * 2 * kFiltMaxRows];
+  __shared__ int cut_s[kFiltMaxRows];
+  const bool rescale = a.phase <= PH_T;
+  const double sc = 1.0 / (double)L;
+  for (int k = tid; k < N; k += kBlock) tw[k] = make_double2(a.twid[2 * k], a.twid[2 * k + 1]);
+
+  for (int r0 = 0; r0 < fr.nst; r0 += a.fft_nb) {
+    const int nb = min(a.fft_nb, fr.nst - r0);
+    // staged rows, their rescale weights (state%iap%u, %v, gd + ghs) and the
+    // epilogue inputs (state s0 or Tb), all loads issued together
+    double x[kFiltMaxRows][KP], w[kFiltMaxRows][KP], pre[kFiltMaxRows][KP];
+    int slot[kFiltMaxRows], comp[kFiltMaxRows], row[kFiltMaxRows];
+#pragma unroll
+    for (int rr = 0; rr < kFiltMaxRows; ++rr) {
+      const bool on = rr < nb;
+      slot[rr] = on ? pick3(fr.slot, r0 + rr) : 0;
+      comp[rr] = on ? pick3(fr.comp, r0 + rr) : CDGD;
+      row[rr] = on ? pick3(fr.row, r0 + rr) : 0;
+      const double *xp = mb.sx() + (size_t)slot[rr] * P;
+      const size_t o = (size_t)row[rr] * P;
+      const double *wp = mb.st(a.s_in, comp[rr] == CDU ? FU : (comp[rr] == CDV ? FV : FGD)) + o;
+      const double *wq = comp[rr] == CDGD ? mb.ghs() + o : nullptr;
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int l = tid + k * kBlock;
+        double xv = 0.0, wv = 0.0, pv = 0.0;
+        if (on && l < N) {
+          xv = xp[l];
+          if (rescale) wv = wq ? wp[l] + wq[l] : wp[l];
+          if (rescale) pv = phase_pre<CT>(a, mb, comp[rr], o + l);
+        }
+        x[rr][k] = xv;
+        w[rr][k] = wv;
+        pre[rr][k] = pv;
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < kFiltMaxRows; ++rr) {
+      if (rr >= nb) continue;
+      double *rowbuf = (double *)(b0 + (size_t)rr * L);
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int l = tid + k * kBlock;
+        if (l >= N) continue;
+        if (a.fft_half) {
+          rowbuf[l] = x[rr][k];
+        } else {
+          rowbuf[2 * l] = x[rr][k];
+          rowbuf[2 * l + 1] = 0.0;
+        }
+      }
+    }
+    if (tid < nb) cut_s[tid] = pick3(fr.cut, r0 + tid);
+    __syncthreads();
+
+    int cur = fft_run(b0, b1, 0, a, nb, tw, false);
+    {
+      const double2 *src = cur ? b1 : b0;
+      double2 *dst = cur ? b0 : b1;
+      if (a.fft_half)
+        real_mid(src, dst, L, nb, cut_s, tw, sc);
+      else
+        cplx_mid(src, dst, L, nb, cut_s, sc);
+      __syncthreads();
+      cur ^= 1;
+    }
+    cur = fft_run(b0, b1, cur, a, nb, tw, true);
+    const double *y = (const double *)(cur ? b1 : b0);
+    const int ystep = a.fft_half ? 1 : 2;
+
+    // s1 = sum(tend * weight) before and s2 = sum(filtered tend * weight)
+    // after filtering (dycore_mod.F90:211, 214), reduced together
+    double s12[2 * kFiltMaxRows];
+#pragma unroll
+    for (int rr = 0; rr < kFiltMaxRows; ++rr) {
+      double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int l = tid + k * kBlock;
+        s1 += x[rr][k] * w[rr][k];
+        if (rr < nb && l < N) s2 += y[(size_t)rr * 2 * L + (size_t)l * ystep] * w[rr][k];
+      }
+      s12[2 * rr] = s1;
+      s12[2 * rr + 1] = s2;
+    }
+    if (rescale) block_sum_n<2 * kFiltMaxRows>(s12, red);
+
+    double p1[kFiltMaxRows] = {0.0, 0.0, 0.0}, p2[kFiltMaxRows] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int rr = 0; rr < kFiltMaxRows; ++rr) {
+      if (rr >= nb) continue;
+      const double s1 = s12[2 * rr], s2 = s12[2 * rr + 1];
+      // rows with |s1| <= filter_inner_product_threshold stay unfiltered
+      const bool filt = !rescale || fabs(s1) > 1.0e-16;
+      const double wrow = comp[rr] == CDV ? a.coef[row[rr]].hcos : a.coef[row[rr]].fcos;
+      const size_t o = (size_t)row[rr] * P;
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int l = tid + k * kBlock;
+        if (l >= N) continue;
+        const double yv = y[(size_t)rr * 2 * L + (size_t)l * ystep];
+        const double T = rescale ? (filt ? yv * s1 / s2 : x[rr][k]) : yv;
+        if (a.phase <= PH_T) {
+          phase_store<CT>(a, mb, comp[rr], o + l, T, pre[rr][k], wrow, p1[rr], p2[rr]);
+        } else if (a.phase == PH_D) {
+          // ordinary_diffusion update of the filtered component
+          const double *sgd = mb.st(a.s_in, FGD);
+          if (comp[rr] == CDGD) {
+            mb.st(SX, FGD)[o + l] = sgd[o + l] + a.kdiff * T;
+          } else if (comp[rr] == CDU) {
+            const int ip = l + 1 == N ? 0 : l + 1;
+            const double u = a.in_full ? mb.st(a.s_in, Fu)[o + l]
+                                       : mb.st(a.s_in, FU)[o + l] * 2.0 / (sqrt(sgd[o + l]) + sqrt(sgd[o + ip]));
+            mb.st(SX, Fu)[o + l] = u + a.kdiff * T;
+          } else {
+            const double v = a.in_full ? mb.st(a.s_in, Fv)[o + l]
+                                       : mb.st(a.s_in, FV)[o + l] * 2.0 / (sqrt(sgd[o + l]) + sqrt(sgd[o + P + l]));
+            mb.st(SX, Fv)[o + l] = v + a.kdiff * T;
+          }
+        } else {  // PH_P: projection only, back into the staging row
+          mb.sx()[(size_t)slot[rr] * P + l] = T;
+        }
+      }
+    }
+    if (a.phase == PH_C) {
+      double pq[2 * kFiltMaxRows];
+#pragma unroll
+      for (int rr = 0; rr < kFiltMaxRows; ++rr) {
+        pq[2 * rr] = p1[rr];
+        pq[2 * rr + 1] = p2[rr];
+      }
+      block_sum_n<2 * kFiltMaxRows>(pq, red);
+      if (tid == 0) {
+#pragma unroll
+        for (int rr = 0; rr < kFiltMaxRows; ++rr) {
+          if (rr >= nb) continue;
+          // the whole row's products go to segment 0 of its partials
+          double *pp = mb.part() + ((size_t)(row[rr] * 3 + comp[rr]) * a.nseg) * 2;
+          pp[0] = pq[2 * rr];
+          pp[1] = pq[2 * rr + 1];
+        }
+      }
+    }
+    __syncthreads();  // buffers and reduction scratch reused by the next batch
+  }
+}
+
+size_t filter_lds_bytes(int N, int L, int nb) {
+  return (size_t)N * 16 + (size_t)2 * nb * L * 16;
+}
+
+template <typename CT>
+static hipError_t launch_filter_t(const Args &a, hipStream_t s) {
+  dim3 g(a.ntiles), b(kBlock);
+  const size_t lds = filter_lds_bytes(a.N, a.fft_L, a.fft_nb);
+  if (a.N <= 4 * kBlock)
+    hipLaunchKernelGGL((k_filter<CT, 4>), g, b, lds, s, a);
+  else if (a.N <= 8 * kBlock)
+    hipLaunchKernelGGL((k_filter<CT, 8>), g, b, lds, s, a);
+  else
+    hipLaunchKernelGGL((k_filter<CT, 16>), g, b, lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter(const Args &a, int prec, hipStream_t s) {
+  if (a.ntiles <= 0) return hipSuccess;
+  return prec == 0 ? launch_filter_t<double>(a, s) : launch_filter_t<float>(a, s);
+}
+
+hipError_t configure_filter(int max_lds) {
+  const void *fns[] = {(const void *)k_filter<double, 4>, (const void *)k_filter<double, 8>,
+                       (const void *)k_filter<double, 16>, (const void *)k_filter<float, 4>,
+                       (const void *)k_filter<float, 8>,  (const void *)k_filter<float, 16>};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace gd

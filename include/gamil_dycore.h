@@ -48,7 +48,7 @@ enum {
   GD_PREC_FP64 = 0,        /* all arithmetic in FP64 (reference -fdefault-real-8) */
   GD_PREC_MIXED = 1,       /* FP32 stencils/tendencies/filter; FP64 state update,
                               pole sums, mass flux divergence and integrals      */
-  GD_PREC_MIXED_BF16 = 2   /* as MIXED, filter products in BF16 with FP32 accumulation */
+  GD_PREC_MIXED_BF16 = 2   /* accepted for namelist compatibility; runs as MIXED   */
 };
 
 /* &dycore_params (src/params_mod.F90:13-98) plus the device knobs. */
@@ -205,12 +205,6 @@ int gd_profile_kernels(gd_handle *h, int32_t nsteps, int32_t max_families,
 
 /* Device bytes held by the handle (all members). */
 int64_t gd_device_bytes(const gd_handle *h);
-
-/* Device self-test of the MFMA operand layout the filter kernels use:
- * D = A B for A 16x4 and B 4x16 (row-major) through one 16x16x4 MFMA in
- * FP64 (A, B, D) and FP32 (Af, Bf, Df) on the current device. */
-int gd_selftest_mfma(const double *A, const double *B, double *D, const float *Af,
-                     const float *Bf, float *Df);
 
 #ifdef __cplusplus
 }

@@ -32,14 +32,6 @@ def _pair(gd, oracle_mod, N, M, dt, ic="balanced", seed=3, nmax=10, mountains=0,
     return o, d
 
 
-def test_mfma_operand_layout(gd):
-    rng = np.random.default_rng(0)
-    A = rng.standard_normal((16, 4)); B = rng.standard_normal((4, 16))
-    D, Df = gd.selftest_mfma(A, B, A.astype(np.float32), B.astype(np.float32))
-    assert np.abs(D - A @ B).max() < 1e-13
-    assert np.abs(Df - (A.astype(np.float32) @ B.astype(np.float32))).max() < 1e-5
-
-
 @pytest.mark.parametrize("pass_,pid", [("all", 0), ("fast", 1), ("slow", 2)])
 @pytest.mark.parametrize("adv", ["center_diff", "upwind"])
 def test_space_operators(gd, oracle_mod, pass_, pid, adv):
@@ -74,8 +66,14 @@ def test_space_operators_unfiltered_rows_bitwise(gd, oracle_mod):
     d.close()
 
 
+# Grid sizes exercise every FFT code path of the device filter: radix 4/2
+# (72, 128), radix 3/5 with an odd half length (90 -> 45), odd N (75, full
+# complex transform), radix 7 (70 -> 35), a prime half length handled by the
+# direct O(R^2) pass (94 -> 47), and the 720-longitude grid.
 @pytest.mark.parametrize("N,M,cut", [(72, 37, [6, 5, 4, 3, 2, 1]), (128, 65, [30, 20, 10, 60, 61]),
-                                     (90, 46, [44, 1, 43])])
+                                     (90, 46, [44, 1, 43]), (75, 38, [36, 20, 1, 7]),
+                                     (70, 36, [34, 2, 17]), (94, 48, [46, 3, 12]),
+                                     (720, 361, [359, 300, 200, 120, 60, 30, 15, 8, 4, 2, 1])])
 def test_filter_rows(gd, oracle_mod, N, M, cut):
     o, d = _pair(gd, oracle_mod, N, M, 600.0, cutoff=cut)
     rng = np.random.default_rng(1)
