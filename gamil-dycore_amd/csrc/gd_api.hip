@@ -48,7 +48,7 @@ struct gd_handle {
   std::vector<FiltRow> frows;    // filter workgroups: (member, latitude) with staged rows
   FiltRow *dfrows = nullptr;
   double *dtwid = nullptr;       // [N][2] cos, sin of 2 pi k/N
-  int fft_L = 1, fft_half = 1, fft_nb = 1;
+  int fft_L = 1, fft_half = 1;
   std::vector<int> fft_rad;
   int32_t *dstep = nullptr;
   double *ddiag = nullptr;  // [nmem][cap][3]
@@ -105,7 +105,6 @@ Args base_args(const gd_handle *h) {
   a.twid = h->dtwid;
   a.fft_L = h->fft_L;
   a.fft_half = h->fft_half;
-  a.fft_nb = h->fft_nb;
   a.fft_nrad = (int)h->fft_rad.size();
   for (int k = 0; k < a.fft_nrad; ++k) a.fft_rad[k] = h->fft_rad[k];
   a.step_ctr = h->dstep;
@@ -304,32 +303,29 @@ void build_tables(gd_handle *h, std::vector<RowCoef> &rc) {
     if (g.half_flag[j]) add(CDV, j, g.half_cut[j]);
   for (int j = 0; j < M; ++j)
     if (g.full_flag[j]) add(CDGD, j, g.full_cut[j]);
-  // filter workgroups: the staged rows of one latitude j of one member (du
-  // and dgd at full row j, dv at half row j).  A row's arithmetic does not
-  // depend on the other rows of its workgroup, so a member gives the same
+  // filter workgroups: one per staged row of each member.  A row's
+  // arithmetic does not depend on any other row, so a member gives the same
   // bits alone and inside an ensemble.
   h->frows.clear();
   for (int m = 0; m < h->nmem; ++m)
-    for (int j = 0; j < M; ++j) {
+    for (int sl = 0; sl < (int)h->slots.size(); ++sl) {
+      const StageSlot &ss = h->slots[sl];
       FiltRow f{};
       f.member = m;
-      for (int c = 0; c < 3; ++c)
-        if (rc[j].stage[c] >= 0) {
-          const StageSlot &ss = h->slots[rc[j].stage[c]];
-          f.slot[f.nst] = rc[j].stage[c];
-          f.comp[f.nst] = ss.comp;
-          f.row[f.nst] = ss.row;
-          f.cut[f.nst] = (ss.kept - 2) / 2;
-          ++f.nst;
-        }
-      if (f.nst) h->frows.push_back(f);
+      f.slot = sl;
+      f.comp = ss.comp;
+      f.row = ss.row;
+      f.cut = (ss.kept - 2) / 2;
+      f.wrow = ss.comp == CDV ? rc[ss.row].hcos : rc[ss.row].fcos;
+      h->frows.push_back(f);
     }
 }
 
-// Radix passes of the complex FFT of length L: 4s first, then 2, 3, 5, 7
-// and the remaining prime factors.
+// Radix passes of the complex FFT of length L: 8s first, then 4, 2, 3, 5,
+// 7 and the remaining prime factors.
 std::vector<int> fft_plan(int L) {
   std::vector<int> r;
+  while (L % 8 == 0) { r.push_back(8); L /= 8; }
   while (L % 4 == 0) { r.push_back(4); L /= 4; }
   while (L % 2 == 0) { r.push_back(2); L /= 2; }
   for (int p = 3; L > 1; p += 2)
@@ -436,14 +432,11 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->fft_half = h->N % 2 == 0;
   h->fft_L = h->fft_half ? h->N / 2 : h->N;
   h->fft_rad = fft_plan(h->fft_L);
-  h->fft_nb = 0;
-  for (int nb = kFiltMaxRows; nb >= 1 && !h->fft_nb; --nb)
-    if (filter_lds_bytes(h->N, h->fft_L, nb) <= prop.sharedMemPerBlock) h->fft_nb = nb;
-  if (!h->frows.empty() && (h->fft_nb == 0 || h->N > 16 * kBlock || (int)h->fft_rad.size() > kMaxRad)) {
+  if (!h->frows.empty() && (filter_lds_bytes(h->N, h->fft_L) > prop.sharedMemPerBlock ||
+                            h->N > 16 * kBlock || (int)h->fft_rad.size() > kMaxRad)) {
     delete h;
     return fail(nullptr, GD_ERR_NOT_SUPPORTED, "num_lon too large for the polar filter (at most 4096)");
   }
-  if (!h->fft_nb) h->fft_nb = 1;
   configure_kernels((int)prop.sharedMemPerBlock);
   hipGetLastError();
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);

@@ -27,7 +27,6 @@ enum Comp { CDU = 0, CDV = 1, CDGD = 2 };
 enum Phase { PH_A = 0, PH_B = 1, PH_C = 2, PH_T = 3, PH_D = 4, PH_P = 5 };
 
 constexpr int kMaxRad = 24;      // radix passes of the filter FFT plan
-constexpr int kFiltMaxRows = 3;  // staged rows of one latitude (du, dv, dgd)
 
 // Per full row j (0-based).  Half-row entries at index j describe half row j
 // (0 for j = M-1).
@@ -79,18 +78,17 @@ struct StageSlot {
   int32_t pad;
 };
 
-// The staged rows of one latitude j of one member, filtered by one
-// workgroup: du at full row j, dv at half row j, dgd at full row j (those
-// that are flagged), with what the kernel needs of each row.
+// One staged row of one member, filtered by one workgroup, with everything
+// the filter kernel needs to know about it.
 struct FiltRow {
   int32_t member;
-  int32_t nst;                     // staged rows (1..3)
-  int32_t slot[kFiltMaxRows];      // staging slots
-  int32_t comp[kFiltMaxRows];      // CDU / CDV / CDGD
-  int32_t row[kFiltMaxRows];       // full row (du, dgd) or half row (dv)
-  int32_t cut[kFiltMaxRows];       // cutoff wavenumber c
+  int32_t slot;     // staging slot
+  int32_t comp;     // CDU / CDV / CDGD
+  int32_t row;      // full row (du, dgd) or half row (dv)
+  int32_t cut;      // cutoff wavenumber c
+  int32_t pad;
+  double wrow;      // area weight of the row in the beta inner products
 };
-
 
 struct Args {
   char *arena;                 // member m at arena + m * mstride
@@ -103,7 +101,6 @@ struct Args {
   int fft_L;                   // complex FFT length: N/2 (even N) or N
   int fft_half;                // 1: even N, real FFT through a half-length complex FFT
   int fft_nrad;                // radix passes
-  int fft_nb;                  // rows transformed together (LDS budget)
   int fft_rad[kMaxRad];
   int32_t *step_ctr;
   int N, M, pitch, nmem;

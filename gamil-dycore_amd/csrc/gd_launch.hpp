@@ -11,9 +11,8 @@ namespace gd {
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// k_filter: twiddle table (N complex) + two FFT buffers of nb rows of L
-// complex points.
-size_t filter_lds_bytes(int N, int L, int nb);
+// k_filter: twiddle table (N complex) + two FFT buffers of L complex points.
+size_t filter_lds_bytes(int N, int L);
 
 // prec: 0 = FP64 plain, 1 = mixed (FP32 compute type)
 hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStream_t s);
