@@ -90,18 +90,24 @@ def run_gpu(gd, prec, ic, steps, warmup, repeats):
 
 
 def pmc_probe(gd, ic, nsteps):
-    """Workload of the hardware-counter runs: a plain device copy of 256 MiB
-    (its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the byte counters),
-    then nsteps steps of the mixed workload.  Run with GD_NO_GRAPH=1 so each
-    kernel is its own dispatch."""
-    import torch
-    n = 64 << 20                          # 64 Mi float32 = 256 MiB
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    print(f"copy calibration: {n * 4} bytes read and written", flush=True)
+    """Workload of the hardware-counter runs: a plain device-to-device copy of
+    256 MiB through the HIP runtime (its FETCH_SIZE / WRITE_SIZE per dispatch
+    calibrate the byte counters), then nsteps steps of the mixed workload.
+    Run with GD_NO_GRAPH=1 so each kernel is its own dispatch.  The process
+    does not import torch: counter collection then sees every dispatch."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    print(f"copy calibration: {nbytes} bytes read and written", flush=True)
+    hip.hipFree(src)
+    hip.hipFree(dst)
     d, _ = make_dycore(gd, "mixed")
     d.set_state(*ic)
     d.step(nsteps)

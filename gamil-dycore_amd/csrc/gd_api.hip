@@ -113,9 +113,13 @@ Args base_args(const gd_handle *h) {
   a.pitch = h->P;
   a.nmem = h->nmem;
   a.m0 = 0;
-  a.nseg = (h->N + kBlock - 1) / kBlock;
+  a.nseg = (h->N + kSeg - 1) / kSeg;
   a.ntiles = h->nmem * h->M * a.nseg;  // (member, row, segment) tiles
   a.nslots = (int)h->slots.size();
+  // k_update: one row per workgroup for a single run; ensembles share about
+  // 512 workgroups (each forms its member's beta from the partials)
+  a.nmem_launch = h->nmem;
+  a.upd_wpm = std::max(1, std::min(h->M, 512 / h->nmem));
   a.upwind = h->cfg.uv_adv_scheme == GD_ADV_UPWIND;
   a.beta_lon = h->cfg.uv_adv_upwind_lon_beta;
   a.beta_lat = h->cfg.uv_adv_upwind_lat_beta;
@@ -157,8 +161,7 @@ hipError_t enqueue_call(gd_handle *h, double dt, int pass, int s0, int s_out, in
   a.dt = dt;
   a.hdt = dt * 0.5;
   hipError_t e;
-  const int row_tiles = h->nmem * h->M;
-  a.ntiles = row_tiles * a.nseg;  // (member, row, segment) tiles
+  a.ntiles = h->nmem * h->M * a.nseg;  // (member, row, segment) tiles
   // first predictor: T = L(S0), S1 = S0 + dt/2 T
   a.phase = PH_A; a.s_in = s0; a.s0 = s0; a.s_out = SP1;
   e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, s0 == SX, h->stream); });
@@ -175,9 +178,7 @@ hipError_t enqueue_call(gd_handle *h, double dt, int pass, int s0, int s_out, in
   if (e != hipSuccess) return e;
   if ((e = enqueue_filter(h, a)) != hipSuccess) return e;
   a.s_out = s_out;
-  e = run(h, "beta", [&] { return launch_beta(a, h->nmem, h->stream); });
-  if (e != hipSuccess) return e;
-  return run(h, "correct", [&] { return launch_correct(a, h->prec, h->stream); });
+  return run(h, "update", [&] { return launch_update(a, h->prec, h->stream); });
 }
 
 hipError_t enqueue_diffusion(gd_handle *h, int s_in, bool in_full, double dt) {
@@ -291,10 +292,12 @@ void build_tables(gd_handle *h, std::vector<RowCoef> &rc) {
     r.hdlat2 = r.hdlat * r.hdlat;
     r.mass_w = r.fcos * g.dlon * g.dlat;
     r.stage[0] = r.stage[1] = r.stage[2] = -1;
+    r.cut[0] = r.cut[1] = r.cut[2] = 0;
   }
   auto add = [&](int comp, int row, int c) {
     StageSlot s{comp, row, kept_slots(N, c), 0};
     rc[row].stage[comp] = (int)h->slots.size();
+    rc[row].cut[comp] = c;
     h->slots.push_back(s);
   };
   for (int j = 1; j <= M - 2; ++j)
@@ -448,7 +451,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   const size_t fld = (size_t)h->M * h->P;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t nslots = h->slots.size();
-  const size_t a_nseg = (h->N + kBlock - 1) / kBlock;
+  const size_t a_nseg = (h->N + kSeg - 1) / kSeg;
   h->fs = al(fld * 8);
   h->cs = al(fld * h->cts);
   h->off_part = (size_t)(NSLOT * NFLD + 5) * h->fs + 4 * h->cs;

@@ -20,6 +20,8 @@ namespace gd {
 
 constexpr int kBlock = 256;   // threads of the row kernels
 constexpr int kXcd = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kSeg = kBlock - 2;  // longitudes per row tile: with the two halo points
+                                  // every thread loads exactly one LDS slot
 
 enum Slot { SX = 0, SY = 1, SZ = 2, SP1 = 3, SP2 = 4, NSLOT = 5 };
 enum Fld { FGD = 0, FU = 1, FV = 2, Fu = 3, Fv = 4, NFLD = 5 };
@@ -39,7 +41,7 @@ struct RowCoef {
   double fdlon2, fdlat2, hdlon2, hdlat2;
   double mass_w;                       // (fcos*dlon)*dlat
   int32_t stage[3];                    // staging slot of du/dv/dgd, -1 if none
-  int32_t pad;
+  int32_t cut[3];                      // their cutoff wavenumbers
 };
 
 // Per-member arena.  Members sit `mstride` bytes apart in one allocation and
@@ -105,10 +107,11 @@ struct Args {
   int32_t *step_ctr;
   int N, M, pitch, nmem;
   int m0;                      // first member of a row launch
-  int nseg;                    // longitude segments of kBlock points per row
+  int nseg;                    // longitude segments of kSeg points per row
   int ntiles;                  // work items of this launch
   int in_full;                 // k_diffuse: read stored u, v of s_in
   int nslots;
+  int upd_wpm, nmem_launch;    // k_update: workgroups per member, members
   int pass, phase;
   int s_in, s0, s_out;
   int upwind;

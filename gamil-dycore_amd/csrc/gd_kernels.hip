@@ -27,66 +27,79 @@ namespace gd {
 // tendency (space_operators, dycore_mod.F90:184-598) + predictor epilogue
 // ---------------------------------------------------------------------------
 
-// Tile = (member, full row j, segment of kBlock longitudes); one thread per
-// longitude.  Every global load of a tile is issued before its first
-// barrier: the rows of G = sqrt(gd), U, V (and u, v) over the segment plus
-// one halo point on each side go to LDS, the point's own gd/ghs and the
-// epilogue's base values to registers, so a tile costs one round trip to
-// memory.  LDS slot li = i - i0 + 2 holds longitude i (periodic wrap).
+// LDS of one segment: G rows j-1..j+2, U rows j-1..j+1, V half rows
+// j-1..j+1 (double), u rows j, j+1 and v half rows j-1..j+1 (compute type),
+// each over the segment plus one halo point on both sides.
+constexpr int kLW = kBlock + 4;  // LDS slot li = i - i0 + 2 holds longitude i
+template <typename CT>
+__host__ __device__ constexpr size_t tend_stencil_bytes() {
+  return (size_t)(10 * sizeof(double) + 5 * sizeof(CT)) * kLW;
+}
+
+// Tendency values of one point and the epilogue inputs loaded with them.
+struct TendPoint {
+  double du, dv, dgd;
+  double pre_u, pre_v, pre_g;
+};
+
+// Segment i0 .. i0+W-1 of full row j: every global load is issued before
+// the first barrier (halo rows to LDS, the point's own geopotential terms
+// and epilogue inputs to registers), then thread tid < W evaluates its
+// point.  smem holds tend_stencil_bytes<CT>() bytes, red 24 doubles.
 template <typename CT, int PASS, bool IN_FULL>
-__global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
-  const int t = xcd_tile(a.ntiles);
-  if (t < 0) return;
-  const int N = a.N, M = a.M, P = a.pitch, nseg = a.nseg;
-  const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
-  const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
-  const int m = a.m0 + mloc;
-  const int i0 = g * kBlock, W = min(N - i0, kBlock);
-  constexpr int LW = kBlock + 4;
-  __shared__ double sG[4][LW];  // G rows j-1 .. j+2
-  __shared__ double sU[3][LW];  // U rows j-1 .. j+1
-  __shared__ double sV[3][LW];  // V half rows j-1 .. j+1
-  __shared__ CT su[2][LW];      // u rows j, j+1
-  __shared__ CT sv[3][LW];      // v half rows j-1 .. j+1
-  __shared__ double red[4 * 6];
-  const Member mb = member_at(a, m);
+__device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &mb, const RowCoef &cj, int j,
+                                                  int i0, int W, bool stg_u, bool stg_v, bool stg_g,
+                                                  unsigned char *smem, double *red) {
+  const int N = a.N, M = a.M, P = a.pitch;
+  double(*sG)[kLW] = (double(*)[kLW])smem;
+  double(*sU)[kLW] = (double(*)[kLW])(smem + 4 * kLW * sizeof(double));
+  double(*sV)[kLW] = (double(*)[kLW])(smem + 7 * kLW * sizeof(double));
+  CT(*su)[kLW] = (CT(*)[kLW])(smem + 10 * kLW * sizeof(double));
+  CT(*sv)[kLW] = (CT(*)[kLW])(smem + 10 * kLW * sizeof(double) + 2 * kLW * sizeof(CT));
   const int tid = threadIdx.x;
   const double *gd = mb.st(a.s_in, FGD);
   const double *Ug = mb.st(a.s_in, FU);
   const double *Vg = mb.st(a.s_in, FV);
   const double *ghs = mb.ghs();
   auto wrap = [N](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
-  const RowCoef cj = a.coef[j];
   const bool has_u = j >= 1 && j <= M - 2;
   const bool has_v = j <= M - 2;
-  const int st_du = has_u ? cj.stage[CDU] : -1;
-  const int st_dv = has_v ? cj.stage[CDV] : -1;
-  const int st_dgd = cj.stage[CDGD];
 
-  // ---- loads: halo rows (slots e = tid, tid + kBlock hold point i0-1+e) ----
-  double lg[2][4], lgn[2][2], lU[2][3], lV[2][3], lfu[2][2], lfv[2][3];
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int e = tid + k * kBlock;
+  // pole mass flux sums over the whole ring (dycore_mod.F90:572-596); every
+  // segment of a pole row forms the same sum in the same order.  Pole tiles
+  // finish these loads before issuing the halo loads, so the two sets of
+  // registers are never live together.
+  const bool pole_row = PASS != 2 && (j == 0 || j == M - 1);
+  double pacc = 0.0;
+  if (pole_row) {
+    const int jn = j == 0 ? 1 : M - 2, hr = j == 0 ? 0 : M - 2;
+#pragma unroll 4
+    for (int q = tid; q < N; q += kBlock)
+      pacc += (sqrt(gd[(size_t)j * P + q]) + sqrt(gd[(size_t)jn * P + q])) * Vg[(size_t)hr * P + q];
+  }
+  // ---- loads: halo rows (thread e holds point i0-1+e, e < W+2 <= kBlock) ----
+  double lg[4], lgn[2], lU[3], lV[3], lfu[2], lfv[3];
+  {
+    const int e = tid;
     const bool on = e < W + 2;
     const int p = wrap(i0 - 1 + e), pn = wrap(i0 + e);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = j - 1 + r;
-      lg[k][r] = on && row >= 0 && row <= M - 1 ? gd[(size_t)row * P + p] : 1.0;
+      lg[r] = on && row >= 0 && row <= M - 1 ? gd[(size_t)row * P + p] : 1.0;
     }
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int row = j + r;
-      lgn[k][r] = on && row <= M - 1 ? gd[(size_t)row * P + pn] : 1.0;
-      lfu[k][r] = IN_FULL && on && row <= M - 1 ? mb.st(a.s_in, Fu)[(size_t)row * P + p] : 0.0;
+      lgn[r] = on && row <= M - 1 ? gd[(size_t)row * P + pn] : 1.0;
+      lfu[r] = IN_FULL && on && row <= M - 1 ? mb.st(a.s_in, Fu)[(size_t)row * P + p] : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const int row = j - 1 + r;
-      lU[k][r] = on && row >= 0 && row <= M - 1 ? Ug[(size_t)row * P + p] : 0.0;
-      lV[k][r] = on && row >= 0 && row <= M - 2 ? Vg[(size_t)row * P + p] : 0.0;
-      lfv[k][r] = IN_FULL && on && row >= 0 && row <= M - 2 ? mb.st(a.s_in, Fv)[(size_t)row * P + p] : 0.0;
+      lU[r] = on && row >= 0 && row <= M - 1 ? Ug[(size_t)row * P + p] : 0.0;
+      lV[r] = on && row >= 0 && row <= M - 2 ? Vg[(size_t)row * P + p] : 0.0;
+      lfv[r] = IN_FULL && on && row >= 0 && row <= M - 2 ? mb.st(a.s_in, Fv)[(size_t)row * P + p] : 0.0;
     }
   }
   // the point's own geopotential terms and epilogue inputs
@@ -100,49 +113,35 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
     h_i = ghs[o + i]; h_ip = ghs[o + ip];
     if (has_v) { g_n = gd[o + P + i]; h_n = ghs[o + P + i]; }
   }
-  double pre_u = 0, pre_v = 0, pre_g = 0;
+  TendPoint out{0, 0, 0, 0, 0, 0};
   if (own) {
-    if (st_du < 0) pre_u = phase_pre<CT>(a, mb, CDU, o + i);
-    if (st_dv < 0 && has_v) pre_v = phase_pre<CT>(a, mb, CDV, o + i);
-    if (st_dgd < 0) pre_g = phase_pre<CT>(a, mb, CDGD, o + i);
+    if (!stg_u) out.pre_u = phase_pre<CT>(a, mb, CDU, o + i);
+    if (!stg_v && has_v) out.pre_v = phase_pre<CT>(a, mb, CDV, o + i);
+    if (!stg_g) out.pre_g = phase_pre<CT>(a, mb, CDGD, o + i);
   }
-  // pole mass flux sums over the whole ring (dycore_mod.F90:572-596); every
-  // segment of a pole row forms the same sum in the same order
-  const bool pole_row = PASS != 2 && (j == 0 || j == M - 1);
-  double pacc = 0.0;
-  if (pole_row) {
-    const int jn = j == 0 ? 1 : M - 2, hr = j == 0 ? 0 : M - 2;
-#pragma unroll 4
-    for (int q = tid; q < N; q += kBlock)
-      pacc += (sqrt(gd[(size_t)j * P + q]) + sqrt(gd[(size_t)jn * P + q])) * Vg[(size_t)hr * P + q];
-  }
-
   // ---- LDS rows ----
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int e = tid + k * kBlock;
-    if (e >= W + 2) continue;
-    const int li = e + 1;
+  if (tid < W + 2) {
+    const int li = tid + 1;
     double G[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      G[r] = sqrt(lg[k][r]);
+      G[r] = sqrt(lg[r]);
       sG[r][li] = G[r];
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      sU[r][li] = lU[k][r];
-      sV[r][li] = lV[k][r];
+      sU[r][li] = lU[r];
+      sV[r][li] = lV[r];
     }
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int row = j + r;
       CT val = (CT)0;
       if (IN_FULL) {
-        val = (CT)lfu[k][r];
+        val = (CT)lfu[r];
       } else if (row >= 1 && row <= M - 2) {
         // update_state: u = U * 2 / (G(i) + G(i+1))  (dycore_mod.F90:636-640)
-        val = ((CT)lU[k][r + 1] * (CT)2) / ((CT)G[r + 1] + (CT)sqrt(lgn[k][r]));
+        val = ((CT)lU[r + 1] * (CT)2) / ((CT)G[r + 1] + (CT)sqrt(lgn[r]));
       }
       su[r][li] = val;
     }
@@ -152,9 +151,9 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
       CT val = (CT)0;
       if (hrow >= 0 && hrow <= M - 2) {
         if (IN_FULL)
-          val = (CT)lfv[k][r];
+          val = (CT)lfv[r];
         else  // v = V * 2 / (G(j) + G(j+1))  (dycore_mod.F90:641-645)
-          val = ((CT)lV[k][r] * (CT)2) / ((CT)G[r] + (CT)G[r + 1]);
+          val = ((CT)lV[r] * (CT)2) / ((CT)G[r] + (CT)G[r + 1]);
       }
       sv[r][li] = val;
     }
@@ -167,6 +166,7 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
   } else {
     __syncthreads();
   }
+  if (!own) return out;
 
   const double hc_m = j >= 1 ? a.coef[j - 1].hcos : 0.0;
   const double hc_0 = has_v ? cj.hcos : 0.0;
@@ -175,133 +175,154 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
   const double c_p = j + 1 <= M - 1 ? a.coef[j + 1].c : 0.0;
   const CT bl = (CT)a.beta_lon, bt = (CT)a.beta_lat;
   const bool upwind = a.upwind != 0;
-  double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
-  double *sx = mb.sx();
-
-  if (own) {
-    const int li = tid + 2;
-    // ---- u equation (full row j) ----
-    double du = 0.0;
-    if (has_u) {
-      const CT u0 = su[0][li], um = su[0][li - 1], up = su[0][li + 1];
-      CT ual = 0, uat = 0, fv = 0, upgf = 0;
-      if (PASS != 1) {
-        const CT U0 = (CT)sU[1][li], Um = (CT)sU[1][li - 1], Up = (CT)sU[1][li + 1];
-        const CT u1 = u0 + um, u2 = u0 + up;
-        const CT kl = (CT)cj.k_uadv_lon;
-        if (!upwind)
-          ual = kl * (u2 * Up - u1 * Um);
-        else
-          ual = kl * (u2 * (U0 + Up) - bl * fabs(u2) * (Up - U0) - u1 * (U0 + Um) +
-                      bl * fabs(u1) * (U0 - Um) - (u2 - u1) * U0);
-        const CT v1 = (sv[0][li] + sv[0][li + 1]) * (CT)hc_m;
-        const CT v2 = (sv[1][li] + sv[1][li + 1]) * (CT)hc_0;
-        const CT Ujp = (CT)sU[2][li], Ujm = (CT)sU[0][li];
-        const CT kt = (CT)cj.k_uadv_lat;
-        if (!upwind)
-          uat = kt * (v2 * Ujp - v1 * Ujm);
-        else
-          uat = kt * (v2 * (U0 + Ujp) - bt * fabs(v2) * (Ujp - U0) - v1 * (U0 + Ujm) +
-                      bt * fabs(v1) * (U0 - Ujm) - (v2 - v1) * U0);
-      }
-      if (PASS != 2) {
-        const CT Vm0 = (CT)sV[0][li], Vm1 = (CT)sV[0][li + 1];
-        const CT V00 = (CT)sV[1][li], V01 = (CT)sV[1][li + 1];
-        fv = (CT)0.25 * ((CT)cj.f + (CT)cj.c * u0) * ((CT)cj.c1 * (Vm0 + Vm1) + (CT)cj.c2 * (V00 + V01));
-        const double dphi = g_ip + h_ip - g_i - h_i;
-        upgf = (CT)0.5 * ((CT)sG[1][li] + (CT)sG[1][li + 1]) / (CT)cj.fdlon * (CT)dphi;
-      }
-      CT d;
-      if (PASS == 0)
-        d = -ual - uat + fv - upgf;
-      else if (PASS == 1)
-        d = fv - upgf;
+  const int li = tid + 2;
+  // ---- u equation (full row j) ----
+  if (has_u) {
+    const CT u0 = su[0][li], um = su[0][li - 1], up = su[0][li + 1];
+    CT ual = 0, uat = 0, fv = 0, upgf = 0;
+    if (PASS != 1) {
+      const CT U0 = (CT)sU[1][li], Um = (CT)sU[1][li - 1], Up = (CT)sU[1][li + 1];
+      const CT u1 = u0 + um, u2 = u0 + up;
+      const CT kl = (CT)cj.k_uadv_lon;
+      if (!upwind)
+        ual = kl * (u2 * Up - u1 * Um);
       else
-        d = -ual - uat;
-      du = (double)d;
-    }
-    // ---- v equation (half row j) ----
-    double dv = 0.0;
-    if (has_v) {
-      CT val = 0, vat = 0, fu = 0, vpgf = 0;
-      if (PASS != 1) {
-        const CT u1 = su[0][li - 1] + su[1][li - 1];
-        const CT u2 = su[0][li] + su[1][li];
-        const CT V0 = (CT)sV[1][li], Vm = (CT)sV[1][li - 1], Vp = (CT)sV[1][li + 1];
-        const CT kl = (CT)cj.k_vadv_lon;
-        if (!upwind)
-          val = kl * (u2 * Vp - u1 * Vm);
-        else
-          val = kl * (u2 * (V0 + Vp) - bl * fabs(u2) * (Vp - V0) - u1 * (V0 + Vm) +
-                      bl * fabs(u1) * (V0 - Vm) - (u2 - u1) * V0);
-        const CT vj = sv[1][li];
-        const CT v1 = vj * (CT)hc_0 + sv[0][li] * (CT)hc_m;
-        const CT v2 = vj * (CT)hc_0 + sv[2][li] * (CT)hc_p;
-        const CT Vjp = (CT)sV[2][li];  // zero beyond the last half row
-        const CT Vjm = (CT)sV[0][li];  // zero before the first half row
-        const CT kt = (CT)cj.k_vadv_lat;
-        if (!upwind)
-          vat = kt * (v2 * Vjp - v1 * Vjm);
-        else
-          vat = kt * (v2 * (V0 + Vjp) - bt * fabs(v2) * (Vjp - V0) - v1 * (V0 + Vjm) +
-                      bt * fabs(v1) * (V0 - Vjm) - (v2 - v1) * V0);
-      }
-      if (PASS != 2) {
-        const CT f0 = (CT)cj.f, c0 = (CT)cj.c, fp = (CT)f_p, cp = (CT)c_p;
-        const CT a1 = (f0 + c0 * su[0][li]) * (CT)sU[1][li];
-        const CT a2 = (f0 + c0 * su[0][li - 1]) * (CT)sU[1][li - 1];
-        const CT a3 = (fp + cp * su[1][li]) * (CT)sU[2][li];
-        const CT a4 = (fp + cp * su[1][li - 1]) * (CT)sU[2][li - 1];
-        fu = (CT)0.25 * (a1 + a2 + a3 + a4);
-        const double dphi = g_n + h_n - g_i - h_i;
-        vpgf = (CT)0.5 * ((CT)sG[1][li] + (CT)sG[2][li]) / (CT)cj.hdlat * (CT)cj.hcos * (CT)dphi;
-      }
-      CT d;
-      if (PASS == 0)
-        d = -val - vat - fu - vpgf;
-      else if (PASS == 1)
-        d = -fu - vpgf;
+        ual = kl * (u2 * (U0 + Up) - bl * fabs(u2) * (Up - U0) - u1 * (U0 + Um) +
+                    bl * fabs(u1) * (U0 - Um) - (u2 - u1) * U0);
+      const CT v1 = (sv[0][li] + sv[0][li + 1]) * (CT)hc_m;
+      const CT v2 = (sv[1][li] + sv[1][li + 1]) * (CT)hc_0;
+      const CT Ujp = (CT)sU[2][li], Ujm = (CT)sU[0][li];
+      const CT kt = (CT)cj.k_uadv_lat;
+      if (!upwind)
+        uat = kt * (v2 * Ujp - v1 * Ujm);
       else
-        d = -val - vat;
-      dv = (double)d;
+        uat = kt * (v2 * (U0 + Ujp) - bt * fabs(v2) * (Ujp - U0) - v1 * (U0 + Ujm) +
+                    bt * fabs(v1) * (U0 - Ujm) - (v2 - v1) * U0);
     }
-    // ---- mass continuity (full row j), FP64 in every mode ----
-    double dgd = 0.0;
     if (PASS != 2) {
-      double mdl = 0.0, mdp;
-      if (j == 0 || j == M - 1) {
-        mdp = pole;
-      } else {
-        const double Gi = sG[1][li], Gip = sG[1][li + 1], Gim = sG[1][li - 1];
-        mdl = ((Gi + Gip) * sU[1][li] - (Gi + Gim) * sU[1][li - 1]) * 0.5 / cj.fdlon;
-        mdp = ((Gi + sG[2][li]) * sV[1][li] * hc_0 - (Gi + sG[0][li]) * sV[0][li] * hc_m) * 0.5 / cj.fdlat;
-      }
-      dgd = -mdl - mdp;
+      const CT Vm0 = (CT)sV[0][li], Vm1 = (CT)sV[0][li + 1];
+      const CT V00 = (CT)sV[1][li], V01 = (CT)sV[1][li + 1];
+      fv = (CT)0.25 * ((CT)cj.f + (CT)cj.c * u0) * ((CT)cj.c1 * (Vm0 + Vm1) + (CT)cj.c2 * (V00 + V01));
+      const double dphi = g_ip + h_ip - g_i - h_i;
+      upgf = (CT)0.5 * ((CT)sG[1][li] + (CT)sG[1][li + 1]) / (CT)cj.fdlon * (CT)dphi;
     }
-    // ---- filter staging or epilogue ----
-    // (pole rows: du = 0 goes through the epilogue like any other value)
-    if (st_du >= 0)
-      sx[(size_t)st_du * P + i] = du;
+    CT d;
+    if (PASS == 0)
+      d = -ual - uat + fv - upgf;
+    else if (PASS == 1)
+      d = fv - upgf;
     else
-      phase_store<CT>(a, mb, CDU, o + i, du, pre_u, cj.fcos, p1[0], p2[0]);
-    if (has_v) {
-      if (st_dv >= 0)
-        sx[(size_t)st_dv * P + i] = dv;
+      d = -ual - uat;
+    out.du = (double)d;
+  }
+  // ---- v equation (half row j) ----
+  if (has_v) {
+    CT val = 0, vat = 0, fu = 0, vpgf = 0;
+    if (PASS != 1) {
+      const CT u1 = su[0][li - 1] + su[1][li - 1];
+      const CT u2 = su[0][li] + su[1][li];
+      const CT V0 = (CT)sV[1][li], Vm = (CT)sV[1][li - 1], Vp = (CT)sV[1][li + 1];
+      const CT kl = (CT)cj.k_vadv_lon;
+      if (!upwind)
+        val = kl * (u2 * Vp - u1 * Vm);
       else
-        phase_store<CT>(a, mb, CDV, o + i, dv, pre_v, cj.hcos, p1[1], p2[1]);
+        val = kl * (u2 * (V0 + Vp) - bl * fabs(u2) * (Vp - V0) - u1 * (V0 + Vm) +
+                    bl * fabs(u1) * (V0 - Vm) - (u2 - u1) * V0);
+      const CT vj = sv[1][li];
+      const CT v1 = vj * (CT)hc_0 + sv[0][li] * (CT)hc_m;
+      const CT v2 = vj * (CT)hc_0 + sv[2][li] * (CT)hc_p;
+      const CT Vjp = (CT)sV[2][li];  // zero beyond the last half row
+      const CT Vjm = (CT)sV[0][li];  // zero before the first half row
+      const CT kt = (CT)cj.k_vadv_lat;
+      if (!upwind)
+        vat = kt * (v2 * Vjp - v1 * Vjm);
+      else
+        vat = kt * (v2 * (V0 + Vjp) - bt * fabs(v2) * (Vjp - V0) - v1 * (V0 + Vjm) +
+                    bt * fabs(v1) * (V0 - Vjm) - (v2 - v1) * V0);
     }
-    if (st_dgd >= 0)
-      sx[(size_t)st_dgd * P + i] = dgd;
+    if (PASS != 2) {
+      const CT f0 = (CT)cj.f, c0 = (CT)cj.c, fp = (CT)f_p, cp = (CT)c_p;
+      const CT a1 = (f0 + c0 * su[0][li]) * (CT)sU[1][li];
+      const CT a2 = (f0 + c0 * su[0][li - 1]) * (CT)sU[1][li - 1];
+      const CT a3 = (fp + cp * su[1][li]) * (CT)sU[2][li];
+      const CT a4 = (fp + cp * su[1][li - 1]) * (CT)sU[2][li - 1];
+      fu = (CT)0.25 * (a1 + a2 + a3 + a4);
+      const double dphi = g_n + h_n - g_i - h_i;
+      vpgf = (CT)0.5 * ((CT)sG[1][li] + (CT)sG[2][li]) / (CT)cj.hdlat * (CT)cj.hcos * (CT)dphi;
+    }
+    CT d;
+    if (PASS == 0)
+      d = -val - vat - fu - vpgf;
+    else if (PASS == 1)
+      d = -fu - vpgf;
     else
-      phase_store<CT>(a, mb, CDGD, o + i, dgd, pre_g, cj.fcos, p1[2], p2[2]);
+      d = -val - vat;
+    out.dv = (double)d;
+  }
+  // ---- mass continuity (full row j), FP64 in every mode ----
+  if (PASS != 2) {
+    double mdl = 0.0, mdp;
+    if (j == 0 || j == M - 1) {
+      mdp = pole;
+    } else {
+      const double Gi = sG[1][li], Gip = sG[1][li + 1], Gim = sG[1][li - 1];
+      mdl = ((Gi + Gip) * sU[1][li] - (Gi + Gim) * sU[1][li - 1]) * 0.5 / cj.fdlon;
+      mdp = ((Gi + sG[2][li]) * sV[1][li] * hc_0 - (Gi + sG[0][li]) * sV[0][li] * hc_m) * 0.5 / cj.fdlat;
+    }
+    out.dgd = -mdl - mdp;
+  }
+  return out;
+}
+
+// Tile = (member, full row j, segment of kSeg longitudes), remapped so each
+// XCD owns a contiguous band of rows.  Components of flagged rows are
+// staged to HBM for the filter kernel, which finishes their epilogue.
+template <typename CT, int PASS, bool IN_FULL>
+__global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[4 * 6];
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const int M = a.M, P = a.pitch, nseg = a.nseg, tid = threadIdx.x;
+  const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
+  const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
+  const Member mb = member_at(a, a.m0 + mloc);
+  const RowCoef cj = a.coef[j];
+  const bool has_u = j >= 1 && j <= M - 2, has_v = j <= M - 2;
+  const int st_u = has_u ? cj.stage[CDU] : -1;
+  const int st_v = has_v ? cj.stage[CDV] : -1;
+  const int st_g = cj.stage[CDGD];
+  const int i0 = g * kSeg, W = min(a.N - i0, kSeg);
+  const TendPoint pt = tend_segment<CT, PASS, IN_FULL>(a, mb, cj, j, i0, W, st_u >= 0, st_v >= 0, st_g >= 0,
+                                                       smem, red);
+  double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
+  if (tid < W) {
+    const int i = i0 + tid;
+    const size_t o = (size_t)j * P + i;
+    // (pole rows: du = 0 goes through the epilogue like any other value)
+    if (st_u >= 0)
+      mb.sx()[(size_t)st_u * P + i] = pt.du;
+    else
+      phase_store<CT>(a, mb, CDU, o, pt.du, pt.pre_u, cj.fcos, p1[0], p2[0]);
+    if (has_v) {
+      if (st_v >= 0)
+        mb.sx()[(size_t)st_v * P + i] = pt.dv;
+      else
+        phase_store<CT>(a, mb, CDV, o, pt.dv, pt.pre_v, cj.hcos, p1[1], p2[1]);
+    }
+    if (st_g >= 0)
+      mb.sx()[(size_t)st_g * P + i] = pt.dgd;
+    else
+      phase_store<CT>(a, mb, CDGD, o, pt.dgd, pt.pre_g, cj.fcos, p1[2], p2[2]);
   }
   if (a.phase == PH_C) {
     double pq[6] = {p1[0], p2[0], p1[1], p2[1], p1[2], p2[2]};
     block_sum_n<6>(pq, red);
-    if (tid < 3) {
-      const int c = tid;
-      const bool staged = c == CDU ? st_du >= 0 : (c == CDV ? st_dv >= 0 : st_dgd >= 0);
-      if (!staged && !(c == CDV && !has_v)) {
+    if (tid == 0) {
+      const int st[3] = {st_u, st_v, st_g};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (st[c] >= 0 || (c == CDV && !has_v)) continue;
         double *pp = mb.part() + ((size_t)(j * 3 + c) * nseg + g) * 2;
         pp[0] = pq[2 * c];
         pp[1] = pq[2 * c + 1];
@@ -365,13 +386,100 @@ __global__ void __launch_bounds__(kBlock) k_correct(const Args a) {
   const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
   const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
   const Member mb = member_at(a, a.m0 + mloc);
-  const int i = g * kBlock + threadIdx.x;
-  if (i >= N) return;
+  const int i = g * kSeg + threadIdx.x;
+  if (threadIdx.x >= kSeg || i >= N) return;
   const double dtb = a.dt * mb.beta()[0];
   const size_t o = (size_t)j * P + i;
   mb.st(a.s_out, FGD)[o] = mb.st(a.s0, FGD)[o] + dtb * ((const double *)mb.tc(CDGD))[o];
   mb.st(a.s_out, FU)[o] = mb.st(a.s0, FU)[o] + dtb * (double)((const CT *)mb.tc(CDU))[o];
   if (j <= M - 2) mb.st(a.s_out, FV)[o] = mb.st(a.s0, FV)[o] + dtb * (double)((const CT *)mb.tc(CDV))[o];
+}
+
+// ---------------------------------------------------------------------------
+// beta + corrector update (predict_correct, dycore_mod.F90:784-790)
+// ---------------------------------------------------------------------------
+
+// upd_wpm workgroups per member.  Each forms the member's beta from the
+// phase-C partials exactly as k_beta does (same loads, same order, so the
+// same bits), then applies S_out = S0 + dt beta Tc to its band of rows.
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) k_update(const Args a) {
+  const int wpm = a.upd_wpm;
+  const int m = a.m0 + blockIdx.x / wpm, w = blockIdx.x - (blockIdx.x / wpm) * wpm;
+  const Member mb = member_at(a, m);
+  __shared__ double red[4 * 6];
+  const int tid = threadIdx.x;
+  const int N = a.N, M = a.M, P = a.pitch, nseg = a.nseg;
+  const int rpw = (M + wpm - 1) / wpm;
+  const int r0 = w * rpw, r1 = min(M, r0 + rpw);
+  const int cnt = (r1 - r0) * N;
+  const double *s0g = mb.st(a.s0, FGD), *s0u = mb.st(a.s0, FU), *s0v = mb.st(a.s0, FV);
+  const double *tcg = (const double *)mb.tc(CDGD);
+  const CT *tcu = (const CT *)mb.tc(CDU), *tcv = (const CT *)mb.tc(CDV);
+  double *og = mb.st(a.s_out, FGD), *ou = mb.st(a.s_out, FU), *ov = mb.st(a.s_out, FV);
+  constexpr int U = 4;  // points per thread per batch
+  size_t off[U];
+  bool on[U], hv[U];
+  double g0[U], u0[U], v0[U], tg[U], tu[U], tv[U];
+  auto load = [&](int base) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int e = base + k * kBlock + tid;
+      on[k] = e < cnt;
+      const int row = r0 + (on[k] ? e / N : 0), i = on[k] ? e - (e / N) * N : 0;
+      hv[k] = on[k] && row <= M - 2;
+      off[k] = (size_t)row * P + i;
+      g0[k] = on[k] ? s0g[off[k]] : 0.0;
+      u0[k] = on[k] ? s0u[off[k]] : 0.0;
+      tg[k] = on[k] ? tcg[off[k]] : 0.0;
+      tu[k] = on[k] ? (double)tcu[off[k]] : 0.0;
+      v0[k] = hv[k] ? s0v[off[k]] : 0.0;
+      tv[k] = hv[k] ? (double)tcv[off[k]] : 0.0;
+    }
+  };
+  // the first batch of the band does not depend on beta: issue it first
+  load(0);
+  // beta from the phase-C partials, the same loads and order as k_beta
+  const int per_comp = M * nseg;
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  for (int k0 = 0; k0 < per_comp; k0 += 8 * kBlock) {
+    double x[8][6];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u * kBlock + tid;
+      const int row = k / nseg, sg = k - (k / nseg) * nseg;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        x[u][2 * c] = x[u][2 * c + 1] = 0.0;
+        if (k < per_comp) {
+          const double *pp = mb.part() + ((size_t)(row * 3 + c) * nseg + sg) * 2;
+          x[u][2 * c] = pp[0];
+          x[u][2 * c + 1] = pp[1];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) q[c] += x[u][c];
+  }
+  block_sum_n<6>(q, red);
+  // du, then dv, then dgd
+  const double ip1 = (q[0] + q[2]) + q[4];
+  const double ip2 = (q[1] + q[3]) + q[5];
+  const double beta = (a.qcon && ip1 != 0.0 && ip2 != 0.0) ? ip1 / ip2 : 1.0;
+  if (w == 0 && tid == 0) mb.beta()[0] = beta;
+  const double dtb = a.dt * beta;
+  for (int base = 0; base < cnt; base += U * kBlock) {
+    if (base) load(base);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (!on[k]) continue;
+      og[off[k]] = g0[k] + dtb * tg[k];
+      ou[off[k]] = u0[k] + dtb * tu[k];
+      if (hv[k]) ov[off[k]] = v0[k] + dtb * tv[k];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -388,7 +496,7 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
   const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
   const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
   const Member mb = member_at(a, a.m0 + mloc);
-  const int i0 = g * kBlock, W = min(N - i0, kBlock);
+  const int i0 = g * kSeg, W = min(N - i0, kSeg);
   constexpr int LW = kBlock + 4;
   __shared__ double sgd[3][LW];  // gd rows j-1 .. j+1
   __shared__ double su[3][LW];   // u rows j-1 .. j+1 (0 on and beyond the poles)
@@ -400,25 +508,25 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
   const double *Vg = mb.st(a.s_in, FV);
   const double *ug = mb.st(a.s_in, Fu), *vg = mb.st(a.s_in, Fv);
   auto wrap = [N](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
-  double lg[2][4], lgn[2][3], lU[2][3], lV[2][3];
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int e = tid + k * kBlock;
+  // thread e holds point i0-1+e (e < W+2 <= kBlock)
+  double lg[4], lgn[3], lU[3], lV[3];
+  {
+    const int e = tid;
     const bool on = e < W + 2;
     const int p = wrap(i0 - 1 + e), pn = wrap(i0 + e);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = j - 1 + r;
-      lg[k][r] = on && row >= 0 && row <= M - 1 ? gd[(size_t)row * P + p] : 1.0;
+      lg[r] = on && row >= 0 && row <= M - 1 ? gd[(size_t)row * P + p] : 1.0;
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const int row = j - 1 + r;
       const bool urow = on && row >= 1 && row <= M - 2;
       const bool vrow = on && row >= 0 && row <= M - 2;
-      lgn[k][r] = urow && !a.in_full ? gd[(size_t)row * P + pn] : 1.0;
-      lU[k][r] = urow ? (a.in_full ? ug[(size_t)row * P + p] : Ug[(size_t)row * P + p]) : 0.0;
-      lV[k][r] = vrow ? (a.in_full ? vg[(size_t)row * P + p] : Vg[(size_t)row * P + p]) : 0.0;
+      lgn[r] = urow && !a.in_full ? gd[(size_t)row * P + pn] : 1.0;
+      lU[r] = urow ? (a.in_full ? ug[(size_t)row * P + p] : Ug[(size_t)row * P + p]) : 0.0;
+      lV[r] = vrow ? (a.in_full ? vg[(size_t)row * P + p] : Vg[(size_t)row * P + p]) : 0.0;
     }
   }
   // pole rows of gdd: zonal sums (diffusion_mod.F90:116-133)
@@ -429,20 +537,17 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
 #pragma unroll 4
     for (int q = tid; q < N; q += kBlock) pacc += gd[(size_t)ja * P + q] - gd[(size_t)jb * P + q];
   }
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int e = tid + k * kBlock;
-    if (e >= W + 2) continue;
-    const int li = e + 1;
+  if (tid < W + 2) {
+    const int li = tid + 1;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const int row = j - 1 + r;
-      sgd[r][li] = lg[k][r];
+      sgd[r][li] = lg[r];
       double uv = 0.0, vv = 0.0;
       if (row >= 1 && row <= M - 2)
-        uv = a.in_full ? lU[k][r] : lU[k][r] * 2.0 / (sqrt(lg[k][r]) + sqrt(lgn[k][r]));
+        uv = a.in_full ? lU[r] : lU[r] * 2.0 / (sqrt(lg[r]) + sqrt(lgn[r]));
       if (row >= 0 && row <= M - 2)
-        vv = a.in_full ? lV[k][r] : lV[k][r] * 2.0 / (sqrt(lg[k][r]) + sqrt(lg[k][r + 1]));
+        vv = a.in_full ? lV[r] : lV[r] * 2.0 / (sqrt(lg[r]) + sqrt(lg[r + 1]));
       su[r][li] = uv;
       sv[r][li] = vv;
     }
@@ -530,7 +635,7 @@ __global__ void __launch_bounds__(kBlock) k_finish(const Args a) {
   const Member mb = member_at(a, a.m0 + mloc);
   __shared__ double red[4 * 4];
   const int tid = threadIdx.x;
-  const int i0 = g * kBlock, W = min(N - i0, kBlock);
+  const int i0 = g * kSeg, W = min(N - i0, kSeg);
   const RowCoef cj = a.coef[j];
   const size_t o = (size_t)j * P;
   double *Xgd = mb.st(SX, FGD), *XU = mb.st(SX, FU), *XV = mb.st(SX, FV);
@@ -661,12 +766,18 @@ static inline int row_grid(int ntiles) { return ((ntiles + kXcd - 1) / kXcd) * k
 template <typename CT>
 static hipError_t launch_tend_t(const Args &a, int pass, bool in_full, hipStream_t s) {
   dim3 g(row_grid(a.ntiles)), b(kBlock);
-#define GD_TEND(P, F) hipLaunchKernelGGL((k_tend<CT, P, F>), g, b, 0, s, a)
+  const size_t lds = tend_lds_bytes(a, sizeof(CT));
+#define GD_TEND(P, F) hipLaunchKernelGGL((k_tend<CT, P, F>), g, b, lds, s, a)
   if (pass == 0) { if (in_full) GD_TEND(0, true); else GD_TEND(0, false); }
   else if (pass == 1) { if (in_full) GD_TEND(1, true); else GD_TEND(1, false); }
   else { if (in_full) GD_TEND(2, true); else GD_TEND(2, false); }
 #undef GD_TEND
   return hipGetLastError();
+}
+
+size_t tend_lds_bytes(const Args &a, size_t cts) {
+  (void)a;
+  return cts == 8 ? tend_stencil_bytes<double>() : tend_stencil_bytes<float>();
 }
 
 hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStream_t s) {
@@ -675,6 +786,15 @@ hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStrea
 
 hipError_t launch_beta(const Args &a, int nmem, hipStream_t s) {
   hipLaunchKernelGGL(k_beta, dim3(nmem), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const Args &a, int prec, hipStream_t s) {
+  dim3 g(a.upd_wpm * a.nmem_launch), b(kBlock);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_update<double>, g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_update<float>, g, b, 0, s, a);
   return hipGetLastError();
 }
 

@@ -14,10 +14,16 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // k_filter: twiddle table (N complex) + two FFT buffers of L complex points.
 size_t filter_lds_bytes(int N, int L);
 
+// k_tend: one segment's stencil rows.
+size_t tend_lds_bytes(const Args &a, size_t cts);
+
 // prec: 0 = FP64 plain, 1 = mixed (FP32 compute type)
 hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStream_t s);
 hipError_t launch_beta(const Args &a, int nmem, hipStream_t s);
 hipError_t launch_correct(const Args &a, int prec, hipStream_t s);
+// beta of predict_correct and the corrector update in one launch
+// (upd_wpm workgroups for each of nmem_launch members)
+hipError_t launch_update(const Args &a, int prec, hipStream_t s);
 hipError_t launch_diffuse(const Args &a, int prec, hipStream_t s);
 // mode 0: iap_transform of X after diffusion; 1: X <- slot s_in with u, v
 // derived; 2: diagnostics of X only.  All modes write the mass/energy
