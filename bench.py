@@ -32,32 +32,35 @@ IC = dict(seed=20241, nmax=20, slope=-3.0, rms_height=150.0, mean_depth=8000.0,
 MODEL_DAYS = 10
 
 
-def algorithmic_bytes_per_step(N, M, S, prec):
-    """Algorithmic HBM bytes of the tendency kernel family per step.
-
-    One predict_correct call runs the tendency kernel three times:
-      A: read S0 (gd, U, V) + ghs, write S1 (gd, U, V)                 7 field passes
-      B: read S1 + ghs + S0, write S2 + Tb (du, dv, dgd)               13
-      C: read S2 + ghs + Tb, write Tc                                  10
-    The slow pass has no pressure gradient and skips ghs (6, 12, 9).
-    A csp2 step has 2 slow and S fast calls.  State fields are FP64; in mixed
-    mode du/dv tendencies are FP32 (dgd stays FP64).
-    """
-    full = N * M            # u, gd, U points
-    half = N * (M - 1)      # v, V points
-    st = 8 * (full + full + half)              # gd, U, V
-    ghs = 8 * full
+def tend_launch_bytes(N, M, prec):
+    """Algorithmic HBM bytes of one launch of the tendency kernel in the
+    second predictor of a fast-pass call (S2 = S0 + dt/2 L(S1), keep Tb):
+    read S1 (gd, U, V), ghs and S0 (gd, U, V); write S2 (gd, U, V) and Tb
+    (du, dv in the compute type, dgd in FP64).  gd, U, u live on N x M full
+    rows, V on N x (M-1) half rows; FP64 state."""
+    full, half = N * M, N * (M - 1)
+    state = 8 * (full + full + half)            # gd, U, V
     tsz = 8 if prec == "fp64" else 4
-    tend = tsz * (full + half) + 8 * full      # du, dv, dgd
+    tend = tsz * (full + half) + 8 * full       # du, dv, dgd
+    return state + 8 * full + state + state + tend
 
-    def call(slow):
-        g = 0 if slow else ghs
-        a = st + g + st
-        b = st + g + st + st + tend
-        c = st + g + tend + tend
-        return a + b + c
 
-    return 2 * call(True) + S * call(False)
+def traffic_from_profile():
+    """HBM bytes per launch of the same kernel launch measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a 256 MiB copy), as
+    committed in profiles/; None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_tend_720x361_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return int(float(vals["traffic_bytes_per_launch"]))
+    except (KeyError, ValueError):
+        return None
 
 
 def make_dycore(gd, prec, members=1):
@@ -85,8 +88,10 @@ def run_gpu(gd, prec, ic, steps, warmup, repeats):
         d.step(int(full))
         lat.append(time.perf_counter() - s0)
     prof = d.profile_kernels(60)
+    tend_us = d.time_kernel("tend", 200)
     d.close()
-    return dict(elapsed=elapsed, mass=mass[:, 0], energy=energy[:, 0], lat=lat, prof=prof, cut=cut)
+    return dict(elapsed=elapsed, mass=mass[:, 0], energy=energy[:, 0], lat=lat, prof=prof, cut=cut,
+                tend_us=tend_us)
 
 
 def pmc_probe(gd, ic, nsteps):
@@ -168,19 +173,19 @@ def main():
     days = args.steps * dt / 86400.0
     val = {p: days / r["elapsed"] for p, r in res.items()}
     mixed = res["mixed"]
-    # roofline of the dominant kernel family (tendency kernel), from event-timed launches
+    # roofline of the tendency kernel: one fast-pass second-predictor launch,
+    # its algorithmic bytes over its duration replayed back to back (HIP
+    # events on the library's stream)
     prof = mixed["prof"]
     dom = max(prof, key=lambda k: prof[k][1])
-    n_launch, ms = prof["tend"]
-    per_step_launch = n_launch / 60.0
-    bytes_step = algorithmic_bytes_per_step(N, M, WORKLOAD["subcycles"], "mixed")
-    avg_ms = ms / n_launch
-    achieved = bytes_step / per_step_launch / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_tend (tendency + predictor epilogue)",
+    nbytes = tend_launch_bytes(N, M, "mixed")
+    avg_s = mixed["tend_us"] * 1e-6
+    achieved = nbytes / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_tend<float, fast pass> (second predictor)",
                 "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": None,
-                "bytes_per_launch": int(bytes_step / per_step_launch),
-                "avg_launch_us": round(avg_ms * 1e3, 3),
+                "frac": round(achieved / 8000.0, 4), "traffic": traffic_from_profile(),
+                "bytes_per_launch": int(nbytes),
+                "avg_launch_us": round(mixed["tend_us"], 3),
                 "dominant_family_by_time": dom,
                 "family_ms_per_step": {k: round(v[1] / 60.0, 4) for k, v in prof.items()}}
     cpu = None

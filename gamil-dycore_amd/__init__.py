@@ -57,7 +57,7 @@ EXPORTS = (
     "gd_get_state", "gd_get_diag_fields", "gd_space_operators", "gd_predict_correct",
     "gd_ordinary_diffusion", "gd_filter_rows", "gd_filter_table", "gd_ic_balanced",
     "gd_ic_rossby_haurwitz", "gd_ic_mountain_zonal_flow", "gd_profile_kernels",
-    "gd_device_bytes", "gd_cutoff_from_lat0",
+    "gd_device_bytes", "gd_cutoff_from_lat0", "gd_time_kernel",
 )
 
 _lib = None
@@ -98,6 +98,7 @@ def lib():
     L.gd_cutoff_from_lat0.argtypes = [C.c_int32, C.c_int32, C.c_double, ip, C.c_int32]
     L.gd_device_bytes.restype = C.c_int64
     L.gd_device_bytes.argtypes = [vp]
+    L.gd_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, dp]
     _lib = L
     return L
 
@@ -265,3 +266,11 @@ class Dycore:
 
     def device_bytes(self):
         return int(lib().gd_device_bytes(self.h))
+
+    def time_kernel(self, kernel, reps=200):
+        """Microseconds per launch of one kernel replayed back to back:
+        kernel 'tend' (fast pass, second predictor), 'filter' or 'update'."""
+        us = C.c_double(0.0)
+        kid = {"tend": 0, "filter": 1, "update": 2}[kernel]
+        _check(lib().gd_time_kernel(self.h, kid, int(reps), C.byref(us)), self.h)
+        return us.value

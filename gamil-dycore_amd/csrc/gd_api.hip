@@ -736,4 +736,52 @@ int gd_profile_kernels(gd_handle *h, int32_t nsteps, int32_t max_families, int32
 
 int64_t gd_device_bytes(const gd_handle *h) { return h ? h->bytes : 0; }
 
+int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us) {
+  if (!h || reps < 1 || !us || kernel < GD_TIME_TEND || kernel > GD_TIME_UPDATE) return GD_ERR_INVALID_ARG;
+  if (kernel == GD_TIME_FILTER && h->frows.empty()) return fail(h, GD_ERR_INVALID_ARG, "no filtered rows");
+  hipSetDevice(h->device);
+  // arguments of the second predictor of a fast-pass call (predict_correct,
+  // dycore_mod.F90:774-777): S2 = S0 + dt/2 L(S1)
+  const gd_config &c = h->cfg;
+  const double dt = c.split_scheme == GD_SPLIT_CSP2 ? c.time_step_size / c.subcycles : c.time_step_size;
+  const int pass = c.split_scheme == GD_SPLIT_CSP2 ? GD_PASS_FAST : GD_PASS_ALL;
+  Args a = base_args(h);
+  a.pass = pass;
+  a.dt = dt;
+  a.hdt = 0.5 * dt;
+  a.ntiles = h->nmem * h->M * a.nseg;
+  a.phase = PH_B;
+  a.s_in = SP1;
+  a.s0 = SY;
+  a.s_out = SP2;
+  if (kernel == GD_TIME_UPDATE) {
+    a.phase = PH_C;
+    a.s_in = SP2;
+    a.s_out = SZ;
+  }
+  hipEvent_t e0, e1;
+  GD_HIP(h, hipEventCreate(&e0));
+  GD_HIP(h, hipEventCreate(&e1));
+  auto one = [&]() -> hipError_t {
+    if (kernel == GD_TIME_TEND) return launch_tend(a, h->prec, pass, false, h->stream);
+    if (kernel == GD_TIME_FILTER) {
+      Args f = a;
+      f.ntiles = (int)h->frows.size();
+      return launch_filter(f, h->prec, h->stream);
+    }
+    return launch_update(a, h->prec, h->stream);
+  };
+  GD_HIP(h, one());  // warm
+  GD_HIP(h, hipEventRecord(e0, h->stream));
+  for (int k = 0; k < reps; ++k) GD_HIP(h, one());
+  GD_HIP(h, hipEventRecord(e1, h->stream));
+  GD_HIP(h, hipEventSynchronize(e1));
+  float ms = 0;
+  GD_HIP(h, hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *us = (double)ms * 1e3 / reps;
+  return GD_OK;
+}
+
 }  // extern "C"

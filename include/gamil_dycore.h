@@ -206,6 +206,15 @@ int gd_profile_kernels(gd_handle *h, int32_t nsteps, int32_t max_families,
 /* Device bytes held by the handle (all members). */
 int64_t gd_device_bytes(const gd_handle *h);
 
+/* Average duration in microseconds of one launch of a single kernel of the
+ * step, replayed `reps` times back to back on the handle's stream between
+ * one pair of HIP events (inputs are whatever the state slots hold; the
+ * state X is not modified).  kernel: GD_TIME_TEND (tendency kernel, fast
+ * pass, second predictor), GD_TIME_FILTER (the filter of that phase),
+ * GD_TIME_UPDATE (beta + corrector). */
+enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2 };
+int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

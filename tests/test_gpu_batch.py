@@ -126,3 +126,25 @@ def test_rejects_bad_state_and_config(gd):
         gd.Dycore(N, M, 600.0, zonal_tend_filter_cutoff_wavenumber=[0, 4])
     with pytest.raises(gd.DycoreError):
         gd.Dycore(N, M, 600.0, zonal_tend_filter_cutoff_wavenumber=[N // 2])
+
+
+def test_time_kernel_leaves_state_untouched(gd):
+    """The measurement entry point replays single kernels on scratch slots
+    only: the state X and the next steps are unchanged."""
+    N, M = 72, 37
+    u, v, h, ghs = gd.ic_balanced(N, M, seed=5, nmax=6)
+    kw = dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_coef=2e5,
+              zonal_tend_filter_cutoff_wavenumber=[6, 5, 4, 3])
+    with gd.Dycore(N, M, 900.0, **kw) as d, gd.Dycore(N, M, 900.0, **kw) as ref:
+        d.set_state(u, v, h, ghs)
+        ref.set_state(u, v, h, ghs)
+        d.step(2)
+        ref.step(2)
+        for k in ("tend", "filter", "update"):
+            assert d.time_kernel(k, 5) > 0.0
+        for a, b in zip(d.get_state(), ref.get_state()):
+            np.testing.assert_array_equal(a, b)
+        ma, ea, _, _ = d.step(2)
+        mb, eb, _, _ = ref.step(2)
+        np.testing.assert_array_equal(ma, mb)
+        np.testing.assert_array_equal(ea, eb)
