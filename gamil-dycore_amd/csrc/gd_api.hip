@@ -189,8 +189,19 @@ hipError_t enqueue_diffusion(gd_handle *h, int s_in, bool in_full, double dt) {
   const int norder = h->cfg.diffusion_order / 2;
   const int sign = ((norder + 1) % 2 == 0) ? 1 : -1;  // (-1)**(order/2+1)
   a.kdiff = sign * dt * h->cfg.diffusion_coef;
-  hipError_t e = run(h, "diffuse", [&] { return launch_diffuse(a, h->prec, h->stream); });
-  if (e != hipSuccess) return e;
+  // diffusion_order/2 applications of the operator; intermediate results
+  // ping-pong between the predictor slots (free after the integrator)
+  hipError_t e;
+  int src = -1;
+  for (int k = 1; k <= norder; ++k) {
+    a.lap_src = src >= 0;
+    a.s_lap = src >= 0 ? src : 0;
+    a.lap_final = k == norder;
+    a.s_lapout = src == SP1 ? SP2 : SP1;
+    e = run(h, "diffuse", [&] { return launch_diffuse(a, h->prec, h->stream); });
+    if (e != hipSuccess) return e;
+    src = a.s_lapout;
+  }
   return enqueue_filter(h, a);
 }
 
@@ -202,6 +213,73 @@ hipError_t enqueue_finish(gd_handle *h, int mode, int s_in, int m0, int nm) {
   hipError_t e = run(h, "finish", [&] { return launch_finish(a, mode, h->stream); });
   if (e != hipSuccess) return e;
   return run(h, "diag", [&] { return launch_diag(a, nm, h->stream); });
+}
+
+// space_operators(state s_in, tend, pass) (dycore_mod.F90:184-365) into Tc,
+// filter and rescale included.
+hipError_t enqueue_tendency(gd_handle *h, int pass, int s_in) {
+  Args a = base_args(h);
+  a.pass = pass;
+  a.phase = PH_T;
+  a.s_in = s_in;
+  a.s0 = s_in;
+  hipError_t e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, s_in == SX, h->stream); });
+  if (e != hipSuccess) return e;
+  return enqueue_filter(h, a);
+}
+
+// one pointwise isp_splitting operation (k_isp mode)
+hipError_t enqueue_isp_op(gd_handle *h, int mode, int s_in, int s_out, double hdt, double scale) {
+  Args a = base_args(h);
+  a.isp_mode = mode;
+  a.s_in = s_in;
+  a.s_out = s_out;
+  a.hdt = hdt;
+  a.isp_scale = scale;
+  return run(h, "isp", [&] { return launch_isp(a, h->prec, h->stream); });
+}
+
+// isp_splitting (dycore_mod.F90:689-752).  The saved state F^n is X itself
+// (never written during the call); the subcycle states ping-pong between Y
+// and Z, the slow-pass state is P1.  Returns the slot of the new state.
+hipError_t enqueue_isp_step(gd_handle *h, int *last) {
+  const gd_config &c = h->cfg;
+  const double dt = c.time_step_size;
+  const double fast_dt = dt / c.subcycles, half_dt = dt * 0.5;
+  const int F = GD_PASS_FAST, SL = GD_PASS_SLOW;
+  hipError_t e;
+#define GD_Q(x) do { if ((e = (x)) != hipSuccess) return e; } while (0)
+  GD_Q(enqueue_tendency(h, SL, SX));                                  // L2(F^n)
+  GD_Q(enqueue_isp_op(h, 6, SX, SX, 0.0, 0.0));                       // accum = 0
+  GD_Q(enqueue_isp_op(h, 0, SX, SX, 0.0, 0.0));                       // slow = L2(F^n)
+  int t1 = SX, t2 = SY;
+  for (int k = 0; k < c.subcycles; ++k) {
+    GD_Q(enqueue_tendency(h, F, t1));
+    GD_Q(enqueue_isp_op(h, 1, t1, t2, fast_dt * 0.5, 0.0));           // P1^{n,k+1/2}
+    GD_Q(enqueue_tendency(h, F, t2));
+    GD_Q(enqueue_isp_op(h, 1, t1, t2, fast_dt * 0.5, 0.0));           // P2^{n,k+1/2}
+    GD_Q(enqueue_tendency(h, F, t2));
+    GD_Q(enqueue_isp_op(h, 2, t1, t2, fast_dt, 0.0));                 // P^{n,k+1}
+    const int nt = t2 == SY ? SZ : SY;
+    t1 = t2;
+    t2 = nt;
+  }
+  GD_Q(enqueue_isp_op(h, 3, SX, SX, 0.0, 2.0 / c.subcycles));         // accum *= 2/m
+  GD_Q(enqueue_tendency(h, SL, t1));                                  // L2(Q^n)
+  GD_Q(enqueue_isp_op(h, 4, t1, SP1, half_dt, 0.0));                  // Q1^{n+1}
+  GD_Q(enqueue_tendency(h, SL, SP1));                                 // L2(Q1^{n+1})
+  GD_Q(enqueue_isp_op(h, 4, t1, SP1, half_dt, 0.0));                  // Q2^{n+1}
+  GD_Q(enqueue_tendency(h, SL, SP1));                                 // L2(Q2^{n+1})
+  GD_Q(enqueue_isp_op(h, 5, SX, SX, 0.0, 0.0));                       // R2^n, beta partials
+#undef GD_Q
+  Args a = base_args(h);
+  a.upd_isp = 1;
+  a.qcon = c.qcon_modified;
+  a.dt = dt;
+  a.s0 = SX;
+  a.s_out = t1 == SY ? SZ : SY;
+  *last = a.s_out;
+  return run(h, "update", [&] { return launch_update(a, h->prec, h->stream); });
 }
 
 // time_integrate (dycore_mod.F90:654-669) + diag_run; leaves the new state in X.
@@ -221,6 +299,8 @@ hipError_t enqueue_step(gd_handle *h) {
     }
     if ((e = enqueue_call(h, 0.5 * dt, GD_PASS_SLOW, t1, t2, c.qcon_modified)) != hipSuccess) return e;
     last = t2;
+  } else if (c.split_scheme == GD_SPLIT_ISP) {
+    if ((e = enqueue_isp_step(h, &last)) != hipSuccess) return e;
   } else {
     // no split (and csp1, which time_integrate does not dispatch)
     if ((e = enqueue_call(h, dt, GD_PASS_ALL, SX, SY, c.qcon_modified)) != hipSuccess) return e;
@@ -395,12 +475,10 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   *out = nullptr;
   std::string v = validate(*cfg, n_members);
   if (!v.empty()) return fail(nullptr, GD_ERR_INVALID_ARG, v);
-  if (cfg->split_scheme == GD_SPLIT_ISP)
-    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "split_scheme 'isp' is not provided on the device path");
   if (cfg->uv_adv_scheme == GD_ADV_WENO)
     return fail(nullptr, GD_ERR_NOT_SUPPORTED, "uv_adv_scheme 'weno' is not provided on the device path");
-  if (cfg->use_diffusion && cfg->diffusion_order != 2)
-    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "only diffusion_order = 2 is provided on the device path");
+  if (cfg->use_diffusion && cfg->diffusion_order < 2)
+    return fail(nullptr, GD_ERR_INVALID_ARG, "diffusion_order must be >= 2");
   gd_handle *h = new gd_handle();
   h->cfg = *cfg;
   h->cutoff.assign(cfg->cutoff, cfg->cutoff + std::max(0, cfg->n_cutoff));
@@ -648,7 +726,7 @@ int gd_predict_correct(gd_handle *h, double dt, int32_t pass, int32_t qcon, doub
 
 int gd_ordinary_diffusion(gd_handle *h, double dt) {
   if (!h) return GD_ERR_INVALID_ARG;
-  if (h->cfg.diffusion_order != 2) return fail(h, GD_ERR_NOT_SUPPORTED, "diffusion_order != 2");
+  if (h->cfg.diffusion_order < 2) return fail(h, GD_ERR_INVALID_ARG, "diffusion_order must be >= 2");
   hipSetDevice(h->device);
   const size_t fb = (size_t)h->M * h->P * 8;
   for (int m = 0; m < h->nmem; ++m)

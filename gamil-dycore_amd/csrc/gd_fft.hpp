@@ -356,11 +356,15 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
   }
   if (a.phase == PH_C) {
     block_sum_n<2>(p, red);
+    // the whole row's products go to segment 0 of its partials, the other
+    // segments of the row hold zero
+    double *pp = mb.part() + ((size_t)(fr.row * 3 + comp) * a.nseg) * 2;
     if (tid == 0) {
-      // the whole row's products go to segment 0 of its partials
-      double *pp = mb.part() + ((size_t)(fr.row * 3 + comp) * a.nseg) * 2;
       pp[0] = p[0];
       pp[1] = p[1];
+    } else if (tid < a.nseg) {
+      pp[2 * tid] = 0.0;
+      pp[2 * tid + 1] = 0.0;
     }
   }
   __syncthreads();  // LDS free for the caller

@@ -414,8 +414,12 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   const int r0 = w * rpw, r1 = min(M, r0 + rpw);
   const int cnt = (r1 - r0) * N;
   const double *s0g = mb.st(a.s0, FGD), *s0u = mb.st(a.s0, FU), *s0v = mb.st(a.s0, FV);
-  const double *tcg = (const double *)mb.tc(CDGD);
+  // the corrector tendency: Tc, or for isp_splitting the combined R held in
+  // the FP64 accumulator (slot SP2 u, v fields and the Tb dgd field)
+  const bool isp = a.upd_isp != 0;
+  const double *tcg = isp ? (const double *)mb.tb(CDGD) : (const double *)mb.tc(CDGD);
   const CT *tcu = (const CT *)mb.tc(CDU), *tcv = (const CT *)mb.tc(CDV);
+  const double *rcu = mb.st(SP2, Fu), *rcv = mb.st(SP2, Fv);
   double *og = mb.st(a.s_out, FGD), *ou = mb.st(a.s_out, FU), *ov = mb.st(a.s_out, FV);
   constexpr int U = 4;  // points per thread per batch
   size_t off[U];
@@ -432,9 +436,9 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
       g0[k] = on[k] ? s0g[off[k]] : 0.0;
       u0[k] = on[k] ? s0u[off[k]] : 0.0;
       tg[k] = on[k] ? tcg[off[k]] : 0.0;
-      tu[k] = on[k] ? (double)tcu[off[k]] : 0.0;
+      tu[k] = on[k] ? (isp ? rcu[off[k]] : (double)tcu[off[k]]) : 0.0;
       v0[k] = hv[k] ? s0v[off[k]] : 0.0;
-      tv[k] = hv[k] ? (double)tcv[off[k]] : 0.0;
+      tv[k] = hv[k] ? (isp ? rcv[off[k]] : (double)tcv[off[k]]) : 0.0;
     }
   };
   // the first batch of the band does not depend on beta: issue it first
@@ -467,9 +471,11 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   // du, then dv, then dgd
   const double ip1 = (q[0] + q[2]) + q[4];
   const double ip2 = (q[1] + q[3]) + q[5];
-  const double beta = (a.qcon && ip1 != 0.0 && ip2 != 0.0) ? ip1 / ip2 : 1.0;
+  double beta = (a.qcon && ip1 != 0.0 && ip2 != 0.0) ? ip1 / ip2 : 1.0;
+  // isp_splitting: beta*4/dt, applied as (dt/2)*beta (dycore_mod.F90:745-750)
+  if (isp) beta = beta * 4.0 / a.dt;
   if (w == 0 && tid == 0) mb.beta()[0] = beta;
-  const double dtb = a.dt * beta;
+  const double dtb = isp ? (a.dt * 0.5) * beta : a.dt * beta;
   for (int base = 0; base < cnt; base += U * kBlock) {
     if (base) load(base);
 #pragma unroll
@@ -483,11 +489,121 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
 }
 
 // ---------------------------------------------------------------------------
+// isp_splitting tendency algebra (dycore_mod.F90:689-752), pointwise
+// ---------------------------------------------------------------------------
+
+// The slow tendency L2(F^n) lives in slot SP2 (dgd, du, dv in its gd, U, V
+// fields) and the accumulated fast tendency in SP2's u, v fields (du, dv)
+// and the Tb dgd field, all FP64.  T is the Tc tendency of the last
+// space_operators launch.  Modes:
+//   ISP_SLOW_SAVE   slow <- T
+//   ISP_FAST        T <- T + slow; S_out <- S_in + h T
+//   ISP_FAST_ACC    acc <- acc + T; T <- T + slow; S_out <- S_in + h T
+//   ISP_ACC_SCALE   acc <- (2/subcycles) acc
+//   ISP_SLOW        T <- T - slow; S_out <- S_in + h T
+//   ISP_FINAL       acc <- (T + slow) + acc; beta partials <acc, F^n> and
+//                   <acc, acc> against the saved state X
+//   ISP_ACC_ZERO    acc <- 0
+// (add_data(a, b) is b <- b + a, types_mod.F90:284-304; update_state,
+// dycore_mod.F90:600-652, with u, v derived where they are read).
+enum IspMode { ISP_SLOW_SAVE = 0, ISP_FAST = 1, ISP_FAST_ACC = 2, ISP_ACC_SCALE = 3, ISP_SLOW = 4, ISP_FINAL = 5,
+               ISP_ACC_ZERO = 6 };
+
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) k_isp(const Args a) {
+  __shared__ double red[4 * 6];
+  const int t = xcd_tile(a.ntiles);
+  if (t < 0) return;
+  const int N = a.N, M = a.M, P = a.pitch, nseg = a.nseg, tid = threadIdx.x;
+  const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
+  const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
+  const Member mb = member_at(a, a.m0 + mloc);
+  const int i0 = g * kSeg, W = min(N - i0, kSeg);
+  const bool hv = j <= M - 2;
+  const int mode = a.isp_mode;
+  const size_t o = (size_t)j * P + i0 + tid;
+  double *sl_g = mb.st(SP2, FGD), *sl_u = mb.st(SP2, FU), *sl_v = mb.st(SP2, FV);
+  double *ac_g = (double *)mb.tb(CDGD), *ac_u = mb.st(SP2, Fu), *ac_v = mb.st(SP2, Fv);
+  double p[6] = {0, 0, 0, 0, 0, 0};
+  if (tid < W) {
+    const double tg = ((const double *)mb.tc(CDGD))[o];
+    const double tu = (double)((const CT *)mb.tc(CDU))[o];
+    const double tv = hv ? (double)((const CT *)mb.tc(CDV))[o] : 0.0;
+    if (mode == ISP_SLOW_SAVE) {
+      sl_g[o] = tg;
+      sl_u[o] = tu;
+      if (hv) sl_v[o] = tv;
+    } else if (mode == ISP_ACC_ZERO) {
+      ac_g[o] = 0.0;
+      ac_u[o] = 0.0;
+      if (hv) ac_v[o] = 0.0;
+    } else if (mode == ISP_ACC_SCALE) {
+      ac_g[o] = a.isp_scale * ac_g[o];
+      ac_u[o] = a.isp_scale * ac_u[o];
+      if (hv) ac_v[o] = a.isp_scale * ac_v[o];
+    } else if (mode == ISP_FINAL) {
+      const double rg = (tg + sl_g[o]) + ac_g[o];
+      const double ru = (tu + sl_u[o]) + ac_u[o];
+      const double rv = hv ? (tv + sl_v[o]) + ac_v[o] : 0.0;
+      ac_g[o] = rg;
+      ac_u[o] = ru;
+      if (hv) ac_v[o] = rv;
+      // inner_product(tend, state) and (tend, tend) (types_mod.F90:347-397)
+      const RowCoef cj = a.coef[j];
+      if (j >= 1 && j <= M - 2) {
+        p[0] = ru * mb.st(SX, FU)[o] * cj.fcos;
+        p[1] = ru * ru * cj.fcos;
+      }
+      if (hv) {
+        p[2] = rv * mb.st(SX, FV)[o] * cj.hcos;
+        p[3] = rv * rv * cj.hcos;
+      }
+      p[4] = rg * mb.st(SX, FGD)[o] * cj.fcos;
+      p[5] = rg * rg * cj.fcos;
+    } else {
+      double Tg, Tu, Tv;
+      if (mode == ISP_SLOW) {
+        Tg = tg - sl_g[o];
+        Tu = tu - sl_u[o];
+        Tv = hv ? tv - sl_v[o] : 0.0;
+      } else {
+        if (mode == ISP_FAST_ACC) {
+          ac_g[o] = ac_g[o] + tg;
+          ac_u[o] = ac_u[o] + tu;
+          if (hv) ac_v[o] = ac_v[o] + tv;
+        }
+        Tg = tg + sl_g[o];
+        Tu = tu + sl_u[o];
+        Tv = hv ? tv + sl_v[o] : 0.0;
+      }
+      const double h = a.hdt;
+      mb.st(a.s_out, FGD)[o] = mb.st(a.s_in, FGD)[o] + h * Tg;
+      mb.st(a.s_out, FU)[o] = mb.st(a.s_in, FU)[o] + h * Tu;
+      if (hv) mb.st(a.s_out, FV)[o] = mb.st(a.s_in, FV)[o] + h * Tv;
+    }
+  }
+  if (mode == ISP_FINAL) {
+    block_sum_n<6>(p, red);
+    if (tid < 3 && !(tid == CDV && !hv)) {
+      double *pp = mb.part() + ((size_t)(j * 3 + tid) * nseg + g) * 2;
+      pp[0] = p[2 * tid];
+      pp[1] = p[2 * tid + 1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // ordinary_diffusion (diffusion_mod.F90:74-217), order 2, FP64 in every mode
 // ---------------------------------------------------------------------------
 
 // Tile = (member, full row j, segment); all loads issued before the barrier
-// like k_tend.  LDS slot li = i - i0 + 2 holds longitude i.
+// like k_tend.  LDS slot li = i - i0 + 2 holds longitude i.  One launch is
+// one application of the second-order operator (diffusion_mod.F90:107-179):
+// diffusion_order/2 applications run in sequence, the first on the state
+// s_in (lap_src = 0), later ones on the previous result held in slot s_lap
+// (gd, u, v fields, lap_src = 1).  Intermediate applications write slot
+// s_lapout (lap_final = 0); the last stages the flagged rows for the filter
+// or adds sign*dt*nu times the result to the state s_in into X.
 template <typename CT>
 __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
   const int t = xcd_tile(a.ntiles);
@@ -507,6 +623,13 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
   const double *Ug = mb.st(a.s_in, FU);
   const double *Vg = mb.st(a.s_in, FV);
   const double *ug = mb.st(a.s_in, Fu), *vg = mb.st(a.s_in, Fv);
+  // operand of this application: the state, or the previous result (raw gd,
+  // u, v, nothing derived)
+  const bool prev = a.lap_src != 0;
+  const bool full = a.in_full || prev;
+  const double *lgd = prev ? mb.st(a.s_lap, FGD) : gd;
+  const double *lu = prev ? mb.st(a.s_lap, Fu) : (a.in_full ? ug : Ug);
+  const double *lv = prev ? mb.st(a.s_lap, Fv) : (a.in_full ? vg : Vg);
   auto wrap = [N](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
   // thread e holds point i0-1+e (e < W+2 <= kBlock)
   double lg[4], lgn[3], lU[3], lV[3];
@@ -517,17 +640,28 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = j - 1 + r;
-      lg[r] = on && row >= 0 && row <= M - 1 ? gd[(size_t)row * P + p] : 1.0;
+      lg[r] = on && row >= 0 && row <= M - 1 ? lgd[(size_t)row * P + p] : 1.0;
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const int row = j - 1 + r;
       const bool urow = on && row >= 1 && row <= M - 2;
       const bool vrow = on && row >= 0 && row <= M - 2;
-      lgn[r] = urow && !a.in_full ? gd[(size_t)row * P + pn] : 1.0;
-      lU[r] = urow ? (a.in_full ? ug[(size_t)row * P + p] : Ug[(size_t)row * P + p]) : 0.0;
-      lV[r] = vrow ? (a.in_full ? vg[(size_t)row * P + p] : Vg[(size_t)row * P + p]) : 0.0;
+      lgn[r] = urow && !full ? gd[(size_t)row * P + pn] : 1.0;
+      lU[r] = urow ? lu[(size_t)row * P + p] : 0.0;
+      lV[r] = vrow ? lv[(size_t)row * P + p] : 0.0;
     }
+  }
+  // the state at the tile's own point for the final update when the operand
+  // is a previous result (u, v derived from U, V exactly as update_state)
+  double x_g = 0.0, x_u = 0.0, x_v = 0.0;
+  if (prev && a.lap_final && tid < W) {
+    const int i = i0 + tid, ip = i + 1 == N ? 0 : i + 1;
+    const size_t o = (size_t)j * P + i;
+    x_g = gd[o];
+    if (j >= 1 && j <= M - 2)
+      x_u = a.in_full ? ug[o] : Ug[o] * 2.0 / (sqrt(gd[o]) + sqrt(gd[o - i + ip]));
+    if (j <= M - 2) x_v = a.in_full ? vg[o] : Vg[o] * 2.0 / (sqrt(gd[o]) + sqrt(gd[o + P]));
   }
   // pole rows of gdd: zonal sums (diffusion_mod.F90:116-133)
   const bool pole_row = j == 0 || j == M - 1;
@@ -535,7 +669,7 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
   if (pole_row) {
     const int ja = j == 0 ? 1 : M - 1, jb = j == 0 ? 0 : M - 2;
 #pragma unroll 4
-    for (int q = tid; q < N; q += kBlock) pacc += gd[(size_t)ja * P + q] - gd[(size_t)jb * P + q];
+    for (int q = tid; q < N; q += kBlock) pacc += lgd[(size_t)ja * P + q] - lgd[(size_t)jb * P + q];
   }
   if (tid < W + 2) {
     const int li = tid + 1;
@@ -545,9 +679,9 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
       sgd[r][li] = lg[r];
       double uv = 0.0, vv = 0.0;
       if (row >= 1 && row <= M - 2)
-        uv = a.in_full ? lU[r] : lU[r] * 2.0 / (sqrt(lg[r]) + sqrt(lgn[r]));
+        uv = full ? lU[r] : lU[r] * 2.0 / (sqrt(lg[r]) + sqrt(lgn[r]));
       if (row >= 0 && row <= M - 2)
-        vv = a.in_full ? lV[r] : lV[r] * 2.0 / (sqrt(lg[r]) + sqrt(lg[r + 1]));
+        vv = full ? lV[r] : lV[r] * 2.0 / (sqrt(lg[r]) + sqrt(lg[r + 1]));
       su[r][li] = uv;
       sv[r][li] = vv;
     }
@@ -602,19 +736,30 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
     else
       vd = vd - (v0 - sv[0][li]) * cj.fcos / cj.hdlat2 * cj.hcos;
   }
+  if (!a.lap_final) {
+    // intermediate application: the result is the next operand (pole rows
+    // of u stay zero; diffusion_mod.F90:172-179)
+    mb.st(a.s_lapout, FGD)[o + i] = gdd;
+    mb.st(a.s_lapout, Fu)[o + i] = ud;
+    if (has_v) mb.st(a.s_lapout, Fv)[o + i] = vd;
+    return;
+  }
+  const double sg = prev ? x_g : g0;
+  const double su0 = prev ? x_u : su[1][li];
+  const double sv0 = prev ? x_v : sv[1][li];
   if (st_g >= 0)
     sx[(size_t)st_g * P + i] = gdd;
   else
-    Xgd[o + i] = g0 + kd * gdd;
+    Xgd[o + i] = sg + kd * gdd;
   if (st_u >= 0)
     sx[(size_t)st_u * P + i] = ud;
   else
-    Xu[o + i] = has_u ? su[1][li] + kd * ud : 0.0;
+    Xu[o + i] = has_u ? su0 + kd * ud : 0.0;
   if (has_v) {
     if (st_v >= 0)
       sx[(size_t)st_v * P + i] = vd;
     else
-      Xv[o + i] = sv[1][li] + kd * vd;
+      Xv[o + i] = sv0 + kd * vd;
   }
 }
 
@@ -786,6 +931,15 @@ hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStrea
 
 hipError_t launch_beta(const Args &a, int nmem, hipStream_t s) {
   hipLaunchKernelGGL(k_beta, dim3(nmem), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_isp(const Args &a, int prec, hipStream_t s) {
+  dim3 g(row_grid(a.ntiles)), b(kBlock);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_isp<double>, g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL(k_isp<float>, g, b, 0, s, a);
   return hipGetLastError();
 }
 

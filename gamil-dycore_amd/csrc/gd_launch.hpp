@@ -24,6 +24,8 @@ hipError_t launch_correct(const Args &a, int prec, hipStream_t s);
 // beta of predict_correct and the corrector update in one launch
 // (upd_wpm workgroups for each of nmem_launch members)
 hipError_t launch_update(const Args &a, int prec, hipStream_t s);
+// isp_splitting tendency algebra and state updates (mode a.isp_mode)
+hipError_t launch_isp(const Args &a, int prec, hipStream_t s);
 hipError_t launch_diffuse(const Args &a, int prec, hipStream_t s);
 // mode 0: iap_transform of X after diffusion; 1: X <- slot s_in with u, v
 // derived; 2: diagnostics of X only.  All modes write the mass/energy

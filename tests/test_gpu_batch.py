@@ -121,7 +121,9 @@ def test_rejects_bad_state_and_config(gd):
         with pytest.raises(gd.DycoreError):
             d.set_state(u, v, -h, ghs)
     with pytest.raises(gd.DycoreError):
-        gd.Dycore(N, M, 600.0, split_scheme="isp")
+        gd.Dycore(N, M, 600.0, uv_adv_scheme="weno")  # not provided (incomplete in the reference)
+    with pytest.raises(gd.DycoreError):
+        gd.Dycore(N, M, 600.0, use_diffusion=True, diffusion_order=1)
     with pytest.raises(gd.DycoreError):
         gd.Dycore(N, M, 600.0, zonal_tend_filter_cutoff_wavenumber=[0, 4])
     with pytest.raises(gd.DycoreError):

@@ -105,10 +105,14 @@ def test_predict_correct(gd, oracle_mod, pass_, pid, dt):
     d.close()
 
 
-def test_ordinary_diffusion(gd, oracle_mod):
+@pytest.mark.parametrize("order", [2, 4, 6])
+def test_ordinary_diffusion(gd, oracle_mod, order):
+    # higher orders apply the second-order operator order/2 times
+    # (diffusion_mod.F90:107-179); the coefficient keeps dt*nu*lap^k small
     N, M = 72, 37
-    o, d = _pair(gd, oracle_mod, N, M, 600.0, use_diffusion=True, diffusion_coef=5e5,
-                 cutoff=[5, 3, 2])
+    nu = {2: 5e5, 4: 5e5 * 4e10, 6: 5e5 * 1.6e21}[order]
+    o, d = _pair(gd, oracle_mod, N, M, 600.0, use_diffusion=True, diffusion_coef=nu,
+                 diffusion_order=order, cutoff=[5, 3, 2])
     o.ordinary_diffusion(600.0, o.old)
     d.ordinary_diffusion(600.0)
     u, v, h = d.get_state()
@@ -135,4 +139,29 @@ def test_steps_vs_oracle(gd, oracle_mod, split):
     assert np.all(np.abs(mass[:, 0] - om) <= 1e-13 * np.abs(om))
     assert np.all(np.abs(energy[:, 0] - oe) <= 1e-12 * np.abs(oe))
     assert np.all(np.abs(beta[:, 0] - ob) <= 1e-10)
+    d.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_isp_steps_vs_oracle(gd, oracle_mod, precision):
+    """isp_splitting (dycore_mod.F90:689-752): slow tendency saved, three fast
+    stages per subcycle with the accumulated fast tendency, two slow
+    corrections, beta = <R,F>/<R,R> * 4/dt applied as (dt/2) beta."""
+    N, M, dt = 72, 37, 900.0
+    kw = dict(split_scheme="isp", subcycles=3, use_diffusion=True, diffusion_coef=2e5,
+              cutoff=[6, 5, 4])
+    o, d0 = _pair(gd, oracle_mod, N, M, dt, mountains=1, **kw)
+    d0.close()
+    u0, v0, h0, ghs = gd.ic_balanced(N, M, seed=3, nmax=10, n_mountains=1, mountain_seed=103)
+    d = gd.Dycore(N, M, dt, zonal_tend_filter_cutoff_wavenumber=kw.pop("cutoff"),
+                  precision=precision, **kw)
+    d.set_state(u0, v0, h0, ghs)
+    om, oe, ob, ns = o.run(3)
+    mass, energy, beta, _ = d.step(3)
+    u, v, h = d.get_state()
+    tol = 1e-11 if precision == "fp64" else 2e-5
+    assert rel_err(h, o.state(o.old, oracle_mod.GD)) < tol
+    assert rel_err(u, o.state(o.old, oracle_mod.U)) < 100 * tol
+    assert rel_err(v, o.state(o.old, oracle_mod.V)) < 100 * tol
+    assert np.all(np.abs(beta[:, 0] - ob) <= 100 * tol * np.abs(ob))
     d.close()
