@@ -42,6 +42,15 @@ class _Config(C.Structure):
     ]
 
 
+class IOInfo(C.Structure):
+    _fields_ = [
+        ("desc", C.c_char_p), ("author", C.c_char_p), ("time_units", C.c_char_p),
+        ("start_time", C.c_char_p), ("time_value", C.c_double), ("time_step_size", C.c_double),
+        ("time_scheme", C.c_char_p), ("split_scheme", C.c_char_p), ("subcycles", C.c_int32),
+        ("restart_time", C.c_char_p), ("elapsed_seconds", C.c_double),
+    ]
+
+
 class ICParams(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64), ("nmax", C.c_int32), ("slope", C.c_double),
@@ -57,7 +66,8 @@ EXPORTS = (
     "gd_get_state", "gd_get_diag_fields", "gd_space_operators", "gd_predict_correct",
     "gd_ordinary_diffusion", "gd_filter_rows", "gd_filter_table", "gd_ic_balanced",
     "gd_ic_rossby_haurwitz", "gd_ic_mountain_zonal_flow", "gd_profile_kernels",
-    "gd_device_bytes", "gd_cutoff_from_lat0", "gd_time_kernel",
+    "gd_device_bytes", "gd_cutoff_from_lat0", "gd_time_kernel", "gd_history_write",
+    "gd_restart_write", "gd_restart_read", "gd_io_last_error",
 )
 
 _lib = None
@@ -99,6 +109,14 @@ def lib():
     L.gd_device_bytes.restype = C.c_int64
     L.gd_device_bytes.argtypes = [vp]
     L.gd_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, dp]
+    L.gd_history_write.argtypes = [C.c_char_p, C.POINTER(IOInfo), C.c_int32, C.c_int32, dp, dp, dp,
+                                   dp, dp, dp, i64, C.c_double, C.c_double]
+    L.gd_restart_write.argtypes = [C.c_char_p, C.POINTER(IOInfo), C.c_int32, C.c_int32, dp, dp, dp,
+                                   dp, i64]
+    L.gd_restart_read.argtypes = [C.c_char_p, C.c_int32, C.c_int32, dp, dp, dp, dp, i64, C.c_char_p,
+                                  C.c_int32, dp]
+    L.gd_io_last_error.restype = C.c_char_p
+    L.gd_io_last_error.argtypes = []
     _lib = L
     return L
 
@@ -156,6 +174,62 @@ def cutoff_from_lat0(num_lon, num_lat, lat0_deg):
 
 
 # ---------------------------------------------------------------------------
+# history and restart files (NetCDF classic, host only)
+# ---------------------------------------------------------------------------
+
+def _b(x):
+    return None if x is None else str(x).encode()
+
+
+def _info(desc="", author="N/A", time_units="days", start_time="0001-01-01T00_00_00",
+          time_value=0.0, time_step_size=0.0, time_scheme="predict_correct", split_scheme="",
+          subcycles=0, restart_time="", elapsed_seconds=0.0):
+    return IOInfo(_b(desc), _b(author), _b(time_units), _b(start_time), float(time_value),
+                  float(time_step_size), _b(time_scheme), _b(split_scheme), int(subcycles),
+                  _b(restart_time), float(elapsed_seconds))
+
+
+def _io_check(code):
+    if code != GD_OK:
+        raise DycoreError(code, lib().gd_io_last_error().decode(errors="replace"))
+
+
+def _dense(a, rows, N):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(rows, N))
+
+
+def write_history(path, u, v, gd, ghs, div, vor, total_energy, total_mass, **info):
+    """history_write_state (history_mod.F90:93-124) of C-grid fields into the
+    NetCDF file `path`; `info` holds the gd_io_info metadata (time_value,
+    time_units, start_time, desc, time_step_size, split_scheme, subcycles ...)."""
+    M, N = np.shape(gd)
+    a = [_dense(u, M, N), _dense(v, M - 1, N), _dense(gd, M, N), _dense(ghs, M, N),
+         _dense(div, M, N), _dense(vor, M - 1, N)]
+    inf = _info(**info)
+    _io_check(lib().gd_history_write(_b(path), C.byref(inf), N, M, *[_dp(x) for x in a], N,
+                                     float(total_energy), float(total_mass)))
+
+
+def write_restart(path, u, v, gd, ghs, **info):
+    """restart_write (restart_mod.F90:57-75) into the NetCDF file `path`."""
+    M, N = np.shape(gd)
+    a = [_dense(u, M, N), _dense(v, M - 1, N), _dense(gd, M, N), _dense(ghs, M, N)]
+    inf = _info(**info)
+    _io_check(lib().gd_restart_write(_b(path), C.byref(inf), N, M, *[_dp(x) for x in a], N))
+
+
+def read_restart(path, num_lon, num_lat):
+    """restart_read (restart_mod.F90:37-55): (u, v, gd, ghs, restart_time, elapsed_seconds)."""
+    N, M = int(num_lon), int(num_lat)
+    u = np.zeros((M, N)); v = np.zeros((M - 1, N)); gd = np.zeros((M, N)); ghs = np.zeros((M, N))
+    rt = C.create_string_buffer(64)
+    el = C.c_double(0.0)
+    _io_check(lib().gd_restart_read(_b(path), N, M, _dp(u), _dp(v), _dp(gd), _dp(ghs), N, rt, 64,
+                                    C.byref(el)))
+    return u, v, gd, ghs, rt.value.decode(), el.value
+
+
+# ---------------------------------------------------------------------------
 # the dycore
 # ---------------------------------------------------------------------------
 
@@ -181,6 +255,7 @@ class Dycore:
         h = C.c_void_p()
         _check(L.gd_create(C.byref(cfg), self.members, C.byref(h)))
         self.h = h
+        self._ghs = {}
 
     def close(self):
         if getattr(self, "h", None):
@@ -201,6 +276,7 @@ class Dycore:
         f = lambda a, r: np.ascontiguousarray(a, dtype=np.float64).reshape(r, self.N)
         u, v, gd, ghs = f(u, self.M), f(v, self.M - 1), f(gd, self.M), f(ghs, self.M)
         _check(lib().gd_set_state(self.h, member, _dp(u), _dp(v), _dp(gd), _dp(ghs), self.N), self.h)
+        self._ghs[member] = ghs.copy()
 
     def get_state(self, member=0):
         u = np.zeros((self.M, self.N)); v = np.zeros((self.M - 1, self.N)); gd = np.zeros((self.M, self.N))
@@ -263,6 +339,19 @@ class Dycore:
                                         names, 1024), self.h)
         fam = names.value.decode().split(",")[: nf.value]
         return {f: (int(launches[k]), float(ms[k])) for k, f in enumerate(fam)}
+
+    # -- output ------------------------------------------------------------
+    def write_history(self, path, member=0, **info):
+        """Write the member's current state as one history file (history_write_state)."""
+        u, v, gd = self.get_state(member)
+        div, vor = self.diag_fields(member)
+        tm, te = self.diag()
+        write_history(path, u, v, gd, self._ghs[member], div, vor, te[member], tm[member], **info)
+
+    def write_restart(self, path, member=0, **info):
+        """Write the member's current state as one restart file (restart_write)."""
+        u, v, gd = self.get_state(member)
+        write_restart(path, u, v, gd, self._ghs[member], **info)
 
     def device_bytes(self):
         return int(lib().gd_device_bytes(self.h))

@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-value",
 ]
-HIP_SOURCES = ["gd_kernels.hip", "gd_filter.hip", "gd_api.hip", "gd_host.cpp"]
+HIP_SOURCES = ["gd_kernels.hip", "gd_filter.hip", "gd_api.hip", "gd_host.cpp", "gd_io.cpp"]
 
 
 def _run(cmd, cwd=None):
@@ -49,7 +49,7 @@ def build_hip():
 
 def build_fortran():
     """dycore_mod-compatible facade + dycore_test driver over the C ABI."""
-    srcs = ["gd_params_mod.F90", "gd_capi_mod.F90", "history_mod.F90", "dycore_mod.F90",
+    srcs = ["gd_params_mod.F90", "gd_capi_mod.F90", "gd_time_mod.F90", "dycore_mod.F90",
             "dycore_test.F90"]
     srcs = [os.path.join(FSRC, s) for s in srcs if os.path.exists(os.path.join(FSRC, s))]
     if not srcs:

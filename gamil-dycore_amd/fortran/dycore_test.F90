@@ -2,8 +2,10 @@
 ! read the namelist named on the command line, dycore_init, write the
 ! initial condition of test_case into the state, dycore_run, dycore_final.
 !
-!   dycore_test.exe <namelist>            run on the GPU
-!   dycore_test.exe <namelist> --check    parse and print the configuration only
+!   dycore_test.exe <namelist>             run on the GPU
+!   dycore_test.exe <namelist> --check     parse and print the configuration only
+!   dycore_test.exe <namelist> --schedule  print the output files the run would
+!                                          write (time manager only, no GPU)
 !
 ! Test cases: rossby_haurwitz_wave (&rossby_haurwitz_wave_test_params R, omg,
 ! gd0), mountain_zonal_flow, and balanced_sh (&balanced_sh_params): a seeded
@@ -19,7 +21,7 @@ program dycore_test
   implicit none
 
   character(256) namelist_file_path, arg
-  logical check_only
+  logical check_only, schedule_only
   integer(c_int) rc
   integer m, ios
   integer(c_int64_t) ld
@@ -42,15 +44,19 @@ program dycore_test
   end if
   call get_command_argument(1, namelist_file_path)
   check_only = .false.
+  schedule_only = .false.
   if (command_argument_count() >= 2) then
     call get_command_argument(2, arg)
     check_only = trim(arg) == '--check'
+    schedule_only = trim(arg) == '--schedule'
   end if
 
   call params_read(namelist_file_path)
 
   if (check_only) then
     call dycore_describe()
+  else if (schedule_only) then
+    call dycore_schedule()
   else
     call run_case()
   end if

@@ -47,6 +47,20 @@ module gd_capi_mod
     real(c_double) :: pert_rms
   end type gd_ic_params
 
+  type, bind(C) :: gd_io_info
+    type(c_ptr) :: desc
+    type(c_ptr) :: author
+    type(c_ptr) :: time_units
+    type(c_ptr) :: start_time
+    real(c_double) :: time_value
+    real(c_double) :: time_step_size
+    type(c_ptr) :: time_scheme
+    type(c_ptr) :: split_scheme
+    integer(c_int32_t) :: subcycles
+    type(c_ptr) :: restart_time
+    real(c_double) :: elapsed_seconds
+  end type gd_io_info
+
   interface
 
     integer(c_int) function gd_create(cfg, n_members, h) bind(C, name='gd_create')
@@ -141,9 +155,81 @@ module gd_capi_mod
       integer(c_int32_t), value :: cap
     end function gd_cutoff_from_lat0
 
+    integer(c_int) function gd_history_write(path, info, num_lon, num_lat, u, v, gd, ghs, div, vor, &
+                                             ld, total_energy, total_mass) bind(C, name='gd_history_write')
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_char, gd_io_info
+      character(kind=c_char), intent(in) :: path(*)
+      type(gd_io_info), intent(in) :: info
+      integer(c_int32_t), value :: num_lon, num_lat
+      real(c_double), intent(in) :: u(*), v(*), gd(*), ghs(*), div(*), vor(*)
+      integer(c_int64_t), value :: ld
+      real(c_double), value :: total_energy, total_mass
+    end function gd_history_write
+
+    integer(c_int) function gd_restart_write(path, info, num_lon, num_lat, u, v, gd, ghs, ld) &
+        bind(C, name='gd_restart_write')
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_char, gd_io_info
+      character(kind=c_char), intent(in) :: path(*)
+      type(gd_io_info), intent(in) :: info
+      integer(c_int32_t), value :: num_lon, num_lat
+      real(c_double), intent(in) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+    end function gd_restart_write
+
+    integer(c_int) function gd_restart_read(path, num_lon, num_lat, u, v, gd, ghs, ld, restart_time, &
+                                            cap, elapsed_seconds) bind(C, name='gd_restart_read')
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: num_lon, num_lat
+      real(c_double), intent(out) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+      character(kind=c_char), intent(out) :: restart_time(*)
+      integer(c_int32_t), value :: cap
+      real(c_double), intent(out) :: elapsed_seconds
+    end function gd_restart_read
+
+    type(c_ptr) function gd_io_last_error() bind(C, name='gd_io_last_error')
+      import :: c_ptr
+    end function gd_io_last_error
+
   end interface
 
 contains
+
+  ! NUL-terminated copy of a Fortran string (for path arguments).
+  function c_string(s) result(res)
+
+    character(*), intent(in) :: s
+    character(kind=c_char) :: res(len_trim(s) + 1)
+
+    integer k
+
+    do k = 1, len_trim(s)
+      res(k) = s(k:k)
+    end do
+    res(len_trim(s) + 1) = c_null_char
+
+  end function c_string
+
+  ! Fortran string of a NUL-terminated C string.
+  function f_string(p, cap) result(res)
+
+    type(c_ptr), intent(in) :: p
+    integer, intent(in) :: cap
+    character(512) :: res
+
+    character(kind=c_char), pointer :: s(:)
+    integer k
+
+    res = ''
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [cap])
+    do k = 1, min(cap, 512)
+      if (s(k) == c_null_char) exit
+      res(k:k) = s(k)
+    end do
+
+  end function f_string
 
   ! Text of gd_last_error as a Fortran string.
   function gd_error_text(h) result(res)
@@ -151,18 +237,7 @@ contains
     type(c_ptr), intent(in) :: h
     character(512) :: res
 
-    type(c_ptr) p
-    character(kind=c_char), pointer :: s(:)
-    integer k
-
-    res = ''
-    p = gd_last_error(h)
-    if (.not. c_associated(p)) return
-    call c_f_pointer(p, s, [512])
-    do k = 1, 512
-      if (s(k) == c_null_char) exit
-      res(k:k) = s(k)
-    end do
+    res = f_string(gd_last_error(h), 512)
 
   end function gd_error_text
 

@@ -8,6 +8,9 @@
 !   ensemble_members        number of device-resident runs of this config
 !   device                  HIP device ordinal
 !   log_every               print the per-step diagnostics line every n steps
+!   write_output            .false.: no history or restart files (timing runs)
+!   output_dir              directory of the history and restart files
+!                           (default: the working directory, as the reference)
 !   days                    accepted as run_days (run/namelist.rh_test:6)
 !   use_zonal_reduce, reduce_adv_lon, zonal_reduce_factors,
 !   use_reduce_tend_smooth  keys of run/namelist.jz_test:16-19 that the
@@ -75,6 +78,8 @@ module gd_params_mod
   integer :: ensemble_members = 1
   integer :: device = 0
   integer :: log_every = 1
+  logical :: write_output = .true.
+  character(256) :: output_dir = ''
   integer :: days = 0
   logical :: use_zonal_reduce = .false.
   logical :: reduce_adv_lon = .false.
@@ -89,6 +94,7 @@ module gd_params_mod
     uv_adv_upwind_lat_beta, use_zonal_tend_filter, zonal_tend_filter_cutoff_wavenumber, &
     use_diffusion, diffusion_order, diffusion_coef, &
     zonal_tend_filter_lat0, precision, ensemble_members, device, log_every, days, &
+    write_output, output_dir, &
     use_zonal_reduce, reduce_adv_lon, zonal_reduce_factors, use_reduce_tend_smooth
 
 contains
@@ -120,13 +126,15 @@ contains
 
   end subroutine params_read
 
-  ! Number of steps: the loop of dycore_mod.F90:131 runs while
-  ! curr_time < end_time (time_mod.F90:138-142), i.e. ceiling(T / dt).
+  ! Number of steps of a run set by run_days/hours/minutes: the loop of
+  ! dycore_mod.F90:131 runs while curr_time < end_time (time_mod.F90:138-142),
+  ! i.e. ceiling(T / dt); run_seconds is not part of end_time
+  ! (time_mod.F90:62-68).
   integer function total_steps() result(n)
 
     real(r8) t
 
-    t = run_days * 86400.0_r8 + run_hours * 3600.0_r8 + run_minutes * 60.0_r8 + run_seconds
+    t = run_days * 86400.0_r8 + run_hours * 3600.0_r8 + run_minutes * 60.0_r8
     if (time_step_size <= 0.0_r8) then
       n = 0
     else

@@ -193,6 +193,63 @@ int gd_ic_mountain_zonal_flow(int32_t num_lon, int32_t num_lat, double *u, doubl
 int gd_cutoff_from_lat0(int32_t num_lon, int32_t num_lat, double lat0_deg, int32_t *cutoff,
                         int32_t cap);
 
+/* ---- history and restart files (host only, no device call) ------------- */
+
+/* Dataset metadata of one output time.  Strings may be NULL (written empty). */
+typedef struct gd_io_info {
+  const char *desc;           /* global attribute desc (case_desc)                   */
+  const char *author;         /* global attribute author                              */
+  const char *time_units;     /* time:units = "<time_units> since <start_time>"      */
+  const char *start_time;     /* start_time_format, e.g. "0001-01-01T00_00_00"       */
+  double time_value;          /* time(1) = elapsed seconds / time_units_in_seconds    */
+  double time_step_size;      /* history: global attribute time_step_size            */
+  const char *time_scheme;    /* history: global attribute time_scheme               */
+  const char *split_scheme;   /* history: global attribute split_scheme              */
+  int32_t subcycles;          /* history: global attribute subcycles                 */
+  const char *restart_time;   /* restart: global attribute restart_time              */
+  double elapsed_seconds;     /* restart: global attribute elapsed_seconds           */
+} gd_io_info;
+
+/* history_write_state (history_mod.F90:93-124) into a new NetCDF classic file
+ * (NF90_CLOBBER, io_mod.F90:417-495) holding one output time: coordinates
+ * lon, lat, ilon, ilat [deg] and time; u and v moved to the A grid
+ * (0.5 (u_i + u_{i-1}), 0.5 (v_j + v_{j-1}) with v_0 = 0) on (lon, lat); gh =
+ * gd + ghs; ghs; vor on (ilon, ilat); div on (lon, lat); te and tm.  Input
+ * fields are the C-grid u (num_lat rows), v (num_lat-1), gd, ghs, div
+ * (num_lat) and vor (num_lat-1) in the host layout above.  Differences from
+ * the reference file, which writes through the count of each declared
+ * dimension: the reference declares vor on (ilon, lat) and div on (lon, ilat)
+ * (history_mod.F90:48-49) and so writes vor with an undefined last row and
+ * div without its north pole row; here both keep their own grids.  The A-grid
+ * v has num_lat-1 defined rows (history_mod.F90:115); its last row holds the
+ * NetCDF double fill value. */
+int gd_history_write(const char *path, const gd_io_info *info, int32_t num_lon, int32_t num_lat,
+                     const double *u, const double *v, const double *gd, const double *ghs,
+                     const double *div, const double *vor, int64_t ld, double total_energy,
+                     double total_mass);
+
+/* restart_write (restart_mod.F90:57-75): u on (ilon, lat), v on (lon, ilat),
+ * gd and ghs on (lon, lat), one time record, global attributes restart_time
+ * and elapsed_seconds.  The interior points are written (the reference hands
+ * the whole halo'd arrays to a write of num_lon x num_lat values,
+ * io_mod.F90:577-621, so its files hold the first elements of the halo'd
+ * storage instead). */
+int gd_restart_write(const char *path, const gd_io_info *info, int32_t num_lon, int32_t num_lat,
+                     const double *u, const double *v, const double *gd, const double *ghs,
+                     int64_t ld);
+
+/* restart_read (restart_mod.F90:37-55): u, v, gd, ghs of the last time record
+ * (any pointer may be NULL) and the global attributes restart_time (copied
+ * into restart_time, capacity cap bytes, may be NULL) and elapsed_seconds (may
+ * be NULL).  Reads NetCDF classic and 64-bit-offset files whose variables are
+ * double. */
+int gd_restart_read(const char *path, int32_t num_lon, int32_t num_lat, double *u, double *v,
+                    double *gd, double *ghs, int64_t ld, char *restart_time, int32_t cap,
+                    double *elapsed_seconds);
+
+/* Text of the last failed history/restart call on this thread. */
+const char *gd_io_last_error(void);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* Runs nsteps steps launching every kernel eagerly with HIP events around

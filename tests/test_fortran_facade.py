@@ -95,7 +95,7 @@ seed = 42, nmax = 8, n_mountains = 1, mountain_seed = 3
 def test_driver_matches_python_path(exe, gd, tmp_path):
     nml = tmp_path / "small.nml"
     nml.write_text(SMALL.format(case="balanced_sh"))
-    r = subprocess.run([exe, str(nml)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, str(nml)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = [ln for ln in r.stdout.splitlines() if ln.startswith(" => ")]
     assert len(lines) == 72 + 1  # initial diag + one line per step
@@ -118,6 +118,6 @@ def test_driver_matches_python_path(exe, gd, tmp_path):
 def test_driver_rossby_haurwitz(exe, tmp_path):
     nml = tmp_path / "rh.nml"
     nml.write_text(SMALL.format(case="rossby_haurwitz_wave") + "&rossby_haurwitz_wave_test_params\n/\n")
-    r = subprocess.run([exe, str(nml)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, str(nml)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "final total_mass" in r.stdout

@@ -1,0 +1,255 @@
+"""History and restart files (history_mod.F90, restart_mod.F90, io_mod.F90)
+and the time/alert manager of the Fortran facade, on the CPU.
+
+The NetCDF writer and reader are the library's own (no NetCDF library in this
+image); scipy.io.netcdf_file is the independent reader/writer they are
+checked against.  Expected schedules are worked out by hand from the alert
+rule of time_mod.F90:196-215 (ring when last_time + period <= curr_time; the
+next time_advance sets last_time to the time it rang) and the period parsing
+of io_mod.F90:192-214.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.io import netcdf_file
+
+from conftest import ROOT
+
+EXE = os.path.join(ROOT, "gamil-dycore_amd", "dycore_test.exe")
+FILL = 9.9692099683868690e36
+
+
+def fields(N, M, seed=0):
+    r = np.random.default_rng(seed)
+    u = r.standard_normal((M, N)); u[0] = u[-1] = 0
+    v = r.standard_normal((M - 1, N))
+    gd = 1e4 + r.standard_normal((M, N))
+    ghs = r.standard_normal((M, N))
+    div = r.standard_normal((M, N))
+    vor = r.standard_normal((M - 1, N))
+    return u, v, gd, ghs, div, vor
+
+
+def read(path):
+    with netcdf_file(str(path), "r", mmap=False) as f:
+        dims = dict(f.dimensions)
+        gatt = {k: (v.decode() if isinstance(v, bytes) else v) for k, v in f._attributes.items()}
+        var = {k: (f.variables[k].dimensions, f.variables[k].data.copy(),
+                   {a: (b.decode() if isinstance(b, bytes) else b)
+                    for a, b in f.variables[k]._attributes.items()})
+               for k in f.variables}
+    return dims, gatt, var
+
+
+def test_history_file_layout(gd, tmp_path):
+    N, M = 12, 7
+    u, v, g, ghs, div, vor = fields(N, M)
+    p = tmp_path / "c.h0.h0.0001-01-01T06:00:00.nc"
+    gd.write_history(str(p), u, v, g, ghs, div, vor, total_energy=3.5, total_mass=-2.25,
+                     desc="a case", time_units="days", start_time="0001-01-01T00_00_00",
+                     time_value=0.25, time_step_size=1200.0, split_scheme="csp2", subcycles=4)
+    assert open(p, "rb").read(4) == b"CDF\x01"  # classic format
+    dims, gatt, var = read(p)
+    assert dims == {"time": None, "lon": N, "lat": M, "ilon": N, "ilat": M - 1}
+    assert gatt["dataset"] == "hist0" and gatt["desc"] == "a case" and gatt["author"] == "N/A"
+    assert gatt["time_scheme"] == "predict_correct" and gatt["split_scheme"] == "csp2"
+    assert gatt["subcycles"] == 4 and gatt["time_step_size"] == 1200.0
+    # coordinates: mesh_mod.F90:70-85 in degrees
+    dims_t, t, tatt = var["time"]
+    assert dims_t == ("time",) and t.tolist() == [0.25]
+    assert tatt["units"] == "days since 0001-01-01T00_00_00"
+    np.testing.assert_allclose(var["lon"][1], np.arange(N) * 360.0 / N, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(var["ilon"][1], (np.arange(N) + 0.5) * 360.0 / N, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(var["lat"][1], -90 + np.arange(M) * 180.0 / (M - 1), rtol=0, atol=1e-12)
+    assert var["lat"][1][-1] == 90.0
+    np.testing.assert_allclose(var["ilat"][1], -90 + (np.arange(M - 1) + 0.5) * 180.0 / (M - 1),
+                               rtol=0, atol=1e-12)
+    assert var["lon"][2]["units"] == "degrees_east" and var["lat"][2]["units"] == "degrees_north"
+    # fields, (time, lat, lon) in C order = (lon, lat, time) in Fortran order
+    for k in ("u", "v", "gh", "ghs", "div"):
+        assert var[k][0] == ("time", "lat", "lon"), k
+    assert var["vor"][0] == ("time", "ilat", "ilon")
+    assert var["te"][0] == ("time",) and var["tm"][0] == ("time",)
+    # A grid: u at lon i from the half points i-1/2 (periodic) and i+1/2
+    np.testing.assert_array_equal(var["u"][1][0], 0.5 * (u + np.roll(u, 1, axis=1)))
+    vA = var["v"][1][0]
+    np.testing.assert_array_equal(vA[0], 0.5 * v[0])  # v_0 = 0 (lat halo)
+    np.testing.assert_array_equal(vA[1:M - 1], 0.5 * (v[1:] + v[:-1]))
+    assert np.all(vA[M - 1] == FILL)
+    np.testing.assert_array_equal(var["gh"][1][0], g + ghs)
+    np.testing.assert_array_equal(var["ghs"][1][0], ghs)
+    np.testing.assert_array_equal(var["div"][1][0], div)
+    np.testing.assert_array_equal(var["vor"][1][0], vor)
+    assert var["te"][1].tolist() == [3.5] and var["tm"][1].tolist() == [-2.25]
+    assert var["u"][2] == {"long_name": "u wind component", "units": "m s-1"}
+    assert var["gh"][2] == {"long_name": "geopotential height", "units": "m2 s-2"}
+
+
+def test_restart_round_trip_bitwise(gd, tmp_path):
+    N, M = 10, 9
+    u, v, g, ghs, _, _ = fields(N, M, seed=3)
+    p = tmp_path / "c.r.0001-01-02T00:00:00.nc"
+    gd.write_restart(str(p), u, v, g, ghs, desc="Restart file", restart_time="0001-01-02T00:00:00",
+                     elapsed_seconds=86400.0, time_value=1.0)
+    dims, gatt, var = read(p)
+    assert gatt["dataset"] == "restart" and gatt["restart_time"] == "0001-01-02T00:00:00"
+    assert gatt["elapsed_seconds"] == 86400.0
+    assert var["u"][0] == ("time", "lat", "ilon") and var["v"][0] == ("time", "ilat", "lon")
+    assert var["gd"][0] == ("time", "lat", "lon") and var["ghs"][0] == ("time", "lat", "lon")
+    np.testing.assert_array_equal(var["u"][1][0], u)
+    np.testing.assert_array_equal(var["v"][1][0], v)
+    u2, v2, g2, ghs2, rt, el = gd.read_restart(str(p), N, M)
+    for a, b in ((u2, u), (v2, v), (g2, g), (ghs2, ghs)):
+        np.testing.assert_array_equal(a, b)
+    assert rt == "0001-01-02T00:00:00" and el == 86400.0
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_restart_reader_reads_independent_files(gd, tmp_path, version):
+    """Files written by scipy (classic and 64-bit offset, two time records,
+    extra variables before the fields) are read by the library's reader: the
+    last record of each field."""
+    N, M = 8, 5
+    recs = [fields(N, M, seed=s) for s in (5, 6)]
+    p = tmp_path / f"scipy_v{version}.nc"
+    with netcdf_file(str(p), "w", version=version) as f:
+        f.restart_time = b"0002-03-04T05:06:07"
+        f.elapsed_seconds = 1234.5
+        f.createDimension("time", None)
+        f.createDimension("lon", N); f.createDimension("lat", M)
+        f.createDimension("ilon", N); f.createDimension("ilat", M - 1)
+        f.createVariable("step", "i", ("time",))[:] = [1, 2]
+        f.createVariable("lon", "d", ("lon",))[:] = np.arange(N)
+        for name, dims, k in (("u", ("time", "lat", "ilon"), 0), ("v", ("time", "ilat", "lon"), 1),
+                              ("gd", ("time", "lat", "lon"), 2), ("ghs", ("time", "lat", "lon"), 3)):
+            var = f.createVariable(name, "d", dims)
+            var[0] = recs[0][k]
+            var[1] = recs[1][k]
+    u, v, g, ghs, rt, el = gd.read_restart(str(p), N, M)
+    for a, k in ((u, 0), (v, 1), (g, 2), (ghs, 3)):
+        np.testing.assert_array_equal(a, recs[1][k])
+    assert rt == "0002-03-04T05:06:07" and el == 1234.5
+
+
+def test_restart_reader_errors(gd, tmp_path):
+    with pytest.raises(gd.DycoreError, match="cannot open"):
+        gd.read_restart(str(tmp_path / "missing.nc"), 8, 5)
+    bad = tmp_path / "bad.nc"
+    bad.write_bytes(b"HDF5 not classic")
+    with pytest.raises(gd.DycoreError, match="not a classic NetCDF"):
+        gd.read_restart(str(bad), 8, 5)
+    p = tmp_path / "small.nc"
+    u, v, g, ghs, _, _ = fields(8, 5)
+    gd.write_restart(str(p), u, v, g, ghs)
+    with pytest.raises(gd.DycoreError, match="wrong size"):
+        gd.read_restart(str(p), 10, 5)
+    trunc = tmp_path / "trunc.nc"
+    trunc.write_bytes(open(p, "rb").read()[:-100])
+    with pytest.raises(gd.DycoreError, match="truncated"):
+        gd.read_restart(str(trunc), 8, 5)
+
+
+# ---- the time manager through the driver's --schedule mode ---------------
+
+BASE = """&dycore_params
+test_case = 'balanced_sh'
+case_name = 'c'
+num_lon = 72
+num_lat = 37
+time_step_size = {dt}
+split_scheme = 'csp2'
+{extra}
+/
+"""
+
+
+@pytest.fixture(scope="module")
+def exe():
+    if not os.path.exists(EXE):
+        pytest.skip("dycore_test.exe not built")
+    return EXE
+
+
+def schedule(exe, tmp_path, dt, extra):
+    nml = tmp_path / "s.nml"
+    nml.write_text(BASE.format(dt=dt, extra=extra))
+    r = subprocess.run([exe, str(nml), "--schedule"], capture_output=True, text=True, timeout=60)
+    ev = []
+    steps = None
+    for ln in r.stdout.splitlines():
+        w = ln.split()
+        if w and w[0] in ("history", "restart"):
+            ev.append((w[0], int(w[1]), float(w[2]), w[3]))
+        elif ln.startswith("steps = "):
+            steps = int(ln.split("=")[1])
+    return r, ev, steps
+
+
+def test_schedule_periods_and_names(exe, tmp_path):
+    r, ev, steps = schedule(exe, tmp_path, 1200, "run_days = 1\nhistory_periods = '6 hours'\n"
+                                                 "restart_period = '12 hours'")
+    assert r.returncode == 0, r.stdout
+    assert steps == 72
+    hist = [e for e in ev if e[0] == "history"]
+    rest = [e for e in ev if e[0] == "restart"]
+    # no output at time 0 for a non-zero period
+    assert [e[1] for e in hist] == [18, 36, 54, 72]
+    assert [e[2] for e in hist] == [0.25, 0.5, 0.75, 1.0]       # days since start
+    assert [e[1] for e in rest] == [36, 72]
+    assert hist[0][3] == "c.h0.h0.0001-01-01T06:00:00.nc"
+    assert rest[1][3] == "c.r.0001-01-02T00:00:00.nc"
+
+
+def test_schedule_ring_resets_to_ring_time(exe, tmp_path):
+    """period 1 hour, dt 40 min: rings at 80 min, then 80 min after each ring
+    (last_time is the time it rang, not last_time + period)."""
+    r, ev, steps = schedule(exe, tmp_path, 2400, "run_hours = 8\nhistory_periods = '1 hours'\n"
+                                                 "restart_period = '100 steps'")
+    assert r.returncode == 0, r.stdout
+    assert steps == 12
+    assert [e[1] for e in ev if e[0] == "history"] == [2, 4, 6, 8, 10, 12]
+    assert [e for e in ev if e[0] == "restart"] == []
+
+
+def test_schedule_once_steps_and_seconds_units(exe, tmp_path):
+    # 'once' is period 0 and rings at every output check, time 0 included;
+    # time_units 'seconds' divides by 60 (io_mod.F90:93-94)
+    r, ev, steps = schedule(exe, tmp_path, 600, "run_hours = 1\nhistory_periods = 'once'\n"
+                                                "restart_period = '2 steps'\ntime_units = 'seconds'")
+    assert r.returncode == 0, r.stdout
+    hist = [e for e in ev if e[0] == "history"]
+    assert [e[1] for e in hist] == [0, 1, 2, 3, 4, 5, 6]
+    assert [e[2] for e in hist] == [k * 600 / 60.0 for k in range(7)]
+    assert hist[0][3] == "c.h0.h0.0001-01-01T00_00_00.nc"   # start stamp format
+    assert hist[1][3] == "c.h0.h0.0001-01-01T00:10:00.nc"   # ISO after the first advance
+    assert [e[1] for e in ev if e[0] == "restart"] == [2, 4, 6]
+
+
+def test_schedule_calendar_from_start_end_time(exe, tmp_path):
+    """start_time/end_time dates (no run_* length) through a leap day."""
+    r, ev, steps = schedule(exe, tmp_path, 3600,
+                            "start_time = 2000, 2, 28, 12, 0\nend_time = 2000, 3, 1, 12, 0\n"
+                            "history_periods = '1 days'\nrestart_period = '2 days'")
+    assert r.returncode == 0, r.stdout
+    assert steps == 48
+    names = [e[3] for e in ev if e[0] == "history"]
+    assert names == ["c.h0.h0.2000-02-29T12:00:00.nc", "c.h0.h0.2000-03-01T12:00:00.nc"]
+    assert [e[3] for e in ev if e[0] == "restart"] == ["c.r.2000-03-01T12:00:00.nc"]
+    r, ev, steps = schedule(exe, tmp_path, 3600,
+                            "start_time = 1999, 12, 31, 0, 0\nend_time = 2001, 1, 1, 0, 0\n"
+                            "history_periods = '366 days'\nrestart_period = '1000 days'")
+    assert steps == 24 * 367
+    assert [e[3] for e in ev if e[0] == "history"] == ["c.h0.h0.2000-12-31T00:00:00.nc"]
+
+
+@pytest.mark.parametrize("extra,msg", [
+    ("run_days = 1\ntime_units = 'minutes'", "Invalid time_units minutes"),
+    ("run_days = 1\nhistory_periods = '2 fortnights'", "Invalid IO period 2 fortnights"),
+    ("run_days = 1\nrestart_period = 'x hours'", "Invalid IO period x hours"),
+])
+def test_schedule_rejects_bad_settings(exe, tmp_path, extra, msg):
+    r, ev, steps = schedule(exe, tmp_path, 1200, extra)
+    assert r.returncode == 1
+    assert "[Error]: " + msg in r.stdout
