@@ -67,7 +67,8 @@ EXPORTS = (
     "gd_ordinary_diffusion", "gd_filter_rows", "gd_filter_table", "gd_ic_balanced",
     "gd_ic_rossby_haurwitz", "gd_ic_mountain_zonal_flow", "gd_profile_kernels",
     "gd_device_bytes", "gd_cutoff_from_lat0", "gd_time_kernel", "gd_history_write",
-    "gd_restart_write", "gd_restart_read", "gd_io_last_error",
+    "gd_restart_write", "gd_restart_read", "gd_io_last_error", "gd_ic_steady_geostrophic",
+    "gd_ic_jet_zonal_flow", "gd_ic_shallow_water_waves",
 )
 
 _lib = None
@@ -103,6 +104,9 @@ def lib():
     L.gd_ic_rossby_haurwitz.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double,
                                         C.c_double, dp, dp, dp, dp, i64]
     L.gd_ic_mountain_zonal_flow.argtypes = [C.c_int32, C.c_int32, dp, dp, dp, dp, i64]
+    L.gd_ic_steady_geostrophic.argtypes = [C.c_int32, C.c_int32, dp, dp, dp, dp, i64]
+    L.gd_ic_jet_zonal_flow.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, dp, dp, dp, i64]
+    L.gd_ic_shallow_water_waves.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, dp, dp, dp, i64, dp]
     L.gd_profile_kernels.argtypes = [vp, C.c_int32, C.c_int32, ip, C.POINTER(C.c_int64), dp,
                                      C.c_char_p, C.c_int32]
     L.gd_cutoff_from_lat0.argtypes = [C.c_int32, C.c_int32, C.c_double, ip, C.c_int32]
@@ -159,6 +163,37 @@ def ic_mountain_zonal_flow(num_lon, num_lat):
     u = np.zeros((M, N)); v = np.zeros((M - 1, N)); gd = np.zeros((M, N)); ghs = np.zeros((M, N))
     _check(lib().gd_ic_mountain_zonal_flow(N, M, _dp(u), _dp(v), _dp(gd), _dp(ghs), N))
     return u, v, gd, ghs
+
+
+def _ic_arrays(N, M):
+    return np.zeros((M, N)), np.zeros((M - 1, N)), np.zeros((M, N)), np.zeros((M, N))
+
+
+def ic_steady_geostrophic(num_lon, num_lat):
+    """Williamson test 2 (steady_geostrophic_flow_test_mod.F90:20-62)."""
+    N, M = int(num_lon), int(num_lat)
+    u, v, gd, ghs = _ic_arrays(N, M)
+    _check(lib().gd_ic_steady_geostrophic(N, M, _dp(u), _dp(v), _dp(gd), _dp(ghs), N))
+    return u, v, gd, ghs
+
+
+def ic_jet_zonal_flow(num_lon, num_lat, perturb=True):
+    """Galewsky barotropic jet (jet_zonal_flow_test_mod.F90:29-105)."""
+    N, M = int(num_lon), int(num_lat)
+    u, v, gd, ghs = _ic_arrays(N, M)
+    _check(lib().gd_ic_jet_zonal_flow(N, M, int(bool(perturb)), _dp(u), _dp(v), _dp(gd), _dp(ghs), N))
+    return u, v, gd, ghs
+
+
+def ic_shallow_water_waves(num_lon, num_lat, wave="rossby"):
+    """Shamir-Paldor analytic wave at time 0 (shallow_water_waves_test_mod.F90):
+    (u, v, gd, ghs, phase_speed [rad/s]); wave 'rossby', 'eig' or 'wig'."""
+    N, M = int(num_lon), int(num_lat)
+    u, v, gd, ghs = _ic_arrays(N, M)
+    c = C.c_double(0.0)
+    flag = {"rossby": 0, "eig": 1, "wig": -1}[wave]
+    _check(lib().gd_ic_shallow_water_waves(N, M, flag, _dp(u), _dp(v), _dp(gd), _dp(ghs), N, C.byref(c)))
+    return u, v, gd, ghs, c.value
 
 
 def cutoff_from_lat0(num_lon, num_lat, lat0_deg):

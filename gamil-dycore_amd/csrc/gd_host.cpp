@@ -2,6 +2,7 @@
 // the initial-condition generators.  Nothing here runs inside the time step.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -400,6 +401,228 @@ extern "C" int gd_ic_mountain_zonal_flow(int32_t N, int32_t M, double *u, double
   if (v)
     for (int j = 0; j < M - 1; ++j)
       for (int i = 0; i < N; ++i) v[(size_t)j * ld + i] = 0.0;
+  return GD_OK;
+}
+
+// steady_geostrophic_flow_test_mod.F90:20-62 (Williamson test 2, alpha = 0):
+// u0 = 2 pi a / 12 days on the interior full rows (0 on the pole rows), v from
+// the reference's expression -u0 cos(lon) sin(alpha) (it names cos(lon)
+// sin_lon), i.e. a signed zero, gd from the balanced height; mesh cos/sin as
+// set by mesh_init (cos = 0 on the pole rows).
+extern "C" int gd_ic_steady_geostrophic(int32_t N, int32_t M, double *u, double *v, double *gdo,
+                                        double *ghs, int64_t ld) {
+  using namespace gd;
+  if (N < 8 || M < 5 || ld < N) return GD_ERR_INVALID_ARG;
+  Geometry g;
+  if (!build_geometry(N, M, false, nullptr, 0, g).empty()) return GD_ERR_INVALID_ARG;
+  const double alpha = 0.0, ca = std::cos(alpha), sa = std::sin(alpha);
+  const double u0 = 2 * kPi * kRadius / (12 * 86400), gd0 = 2.94e4;
+  for (int j = 0; j < M; ++j) {
+    const bool pole = j == 0 || j == M - 1;
+    const double c = pole ? 0.0 : std::cos(g.full_lat[j]), s = g.full_sin[j];
+    for (int i = 0; i < N; ++i) {
+      if (u) u[(size_t)j * ld + i] = pole ? 0.0 : u0 * (c * ca + std::cos(g.half_lon[i]) * s * sa);
+      const double t = s * ca - std::cos(g.full_lon[i]) * c * sa;
+      if (gdo) gdo[(size_t)j * ld + i] = gd0 - (kRadius * kOmega * u0 + (u0 * u0) * 0.5) * (t * t);
+      if (ghs) ghs[(size_t)j * ld + i] = 0.0;
+    }
+  }
+  if (v)
+    for (int j = 0; j < M - 1; ++j)
+      for (int i = 0; i < N; ++i) v[(size_t)j * ld + i] = -u0 * std::cos(g.full_lon[i]) * sa;
+  return GD_OK;
+}
+
+namespace {
+
+// Zonal wind of the barotropically unstable jet (jet_zonal_flow_test_mod.F90:
+// 94-105): u_max / e_n exp(1 / ((lat - lat0)(lat - lat1))) between lat0 = pi/7
+// and lat1 = pi/2 - lat0, zero elsewhere.
+struct Jet {
+  double u_max = 80.0, lat0 = gd::kPi / 7.0, lat1 = gd::kPi / 2.0 - gd::kPi / 7.0;
+  double en = std::exp(-4.0 / ((lat1 - lat0) * (lat1 - lat0)));
+  double u(double lat) const {
+    if (lat <= lat0 || lat >= lat1) return 0.0;
+    return u_max / en * std::exp(1 / (lat - lat0) / (lat - lat1));
+  }
+  // gradient-wind balance d(gh)/dlat = -a u (f + tan(lat) u / a)  (84-92)
+  double integrand(double lat) const {
+    const double uu = u(lat), f = 2 * gd::kOmega * std::sin(lat);
+    return gd::kRadius * uu * (f + std::tan(lat) / gd::kRadius * uu);
+  }
+};
+
+// n-point Gauss-Legendre nodes/weights on [-1, 1] by Newton iteration on the
+// Legendre recurrence (the textbook construction).
+void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(gd::kPi * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+}  // namespace
+
+// jet_zonal_flow_test_mod.F90:29-82 (Galewsky et al. 2004): u from the jet
+// profile on every full row, v = 0, gd = gh0 - int_{-pi/2}^{lat} (gradient
+// wind) on each row (gh0 = g 1e4 on the south pole row), plus, when perturb is
+// nonzero, the Gaussian bump ghd cos(lat) exp(-((lon - pi)/alpha)^2)
+// exp(-((pi/4 - lat)/beta)^2) with ghd = 120 g, alpha = 1/3, beta = 1/15.
+// The integral is evaluated to round-off with composite 16-point
+// Gauss-Legendre over the jet (the integrand vanishes outside (lat0, lat1));
+// the reference calls QUADPACK qags with epsrel = 1e-3, whose result agrees
+// with it to about its own error estimate.
+extern "C" int gd_ic_jet_zonal_flow(int32_t N, int32_t M, int32_t perturb, double *u, double *v,
+                                    double *gdo, double *ghs, int64_t ld) {
+  using namespace gd;
+  if (N < 8 || M < 5 || ld < N) return GD_ERR_INVALID_ARG;
+  Geometry g;
+  if (!build_geometry(N, M, false, nullptr, 0, g).empty()) return GD_ERR_INVALID_ARG;
+  const Jet jet;
+  const double gh0 = kGravity * 1.0e4, ghd = kGravity * 120, lat2 = kPi / 4.0;
+  const double alpha = 1.0 / 3.0, beta = 1.0 / 15.0;
+  std::vector<double> gx, gw;
+  gauss_legendre(16, gx, gw);
+  const int panels = 256;
+  std::vector<double> row(M);
+  for (int j = 0; j < M; ++j) {
+    const double lat = g.full_lat[j];
+    double integral = 0.0;
+    const double b = std::fmin(lat, jet.lat1);
+    if (j > 0 && b > jet.lat0) {
+      const double hw = 0.5 * (b - jet.lat0) / panels;
+      for (int p = 0; p < panels; ++p) {
+        const double mid = jet.lat0 + (2 * p + 1) * hw;
+        double acc = 0.0;
+        for (int q = 0; q < 16; ++q) acc += gw[q] * jet.integrand(mid + hw * gx[q]);
+        integral += acc * hw;
+      }
+    }
+    row[j] = j == 0 ? gh0 : gh0 - integral;
+  }
+  for (int j = 0; j < M; ++j) {
+    const double lat = g.full_lat[j];
+    const double uj = jet.u(lat);
+    for (int i = 0; i < N; ++i) {
+      if (u) u[(size_t)j * ld + i] = uj;
+      double gdv = row[j];
+      if (perturb) {
+        const double x = (g.full_lon[i] - kPi) / alpha, y = (lat2 - lat) / beta;
+        gdv = gdv + ghd * std::cos(lat) * std::exp(-(x * x)) * std::exp(-(y * y));
+      }
+      if (gdo) gdo[(size_t)j * ld + i] = gdv;
+      if (ghs) ghs[(size_t)j * ld + i] = 0.0;
+    }
+  }
+  if (v)
+    for (int j = 0; j < M - 1; ++j)
+      for (int i = 0; i < N; ++i) v[(size_t)j * ld + i] = 0.0;
+  return GD_OK;
+}
+
+namespace {
+
+// Analytic linear waves of Shamir & Paldor (2016, QJRMS), as coded in
+// shallow_water_waves_test_mod.F90:140-305: (n, k) = (5, 10), gH = 5e4,
+// their own Earth constants.
+struct ShamirPaldor {
+  double omega = 7.29212e-5, g = 9.80616, a = 6371220.0, H0 = 5.0e3;
+  double pi = 3.14159265358979323;
+  int n = 5, k = 10;
+  double sigma = 0.5 + std::sqrt(0.25 + 100.0);
+
+  // Phase speed: the roots of k^2 C^3 - E_n C - 2 g H0 Omega / a^2 = 0 by
+  // Cardano's formula (140-178); Rossby (0) = -min |C_j|, EIG (1) = max,
+  // WIG (-1) = min.
+  double phase_speed(int flag) const {
+    const double En = g * H0 / (a * a) * ((n + sigma) * (n + sigma));
+    const double k2 = (double)(k * k), k4 = (double)(k * k * k * k);
+    const double D0 = 3.0 * k2 * En, D4 = -54.0 * k4 * g * H0 * omega / (a * a);
+    double Cj[3];
+    for (int j = 1; j <= 3; ++j) {
+      std::complex<double> d = std::sqrt(std::complex<double>(D4 * D4 - 4.0 * (D0 * D0 * D0), 0.0));
+      d = std::pow(0.5 * (D4 + d), 1.0 / 3.0);
+      d = d * std::exp(std::complex<double>(0.0, 2.0 * pi * j * (1.0 / 3.0)));
+      Cj[j - 1] = (-(1.0 / 3.0) / k2 * (d + D0 / d)).real();
+    }
+    if (flag == 0) return -std::fmin(std::fabs(Cj[0]), std::fmin(std::fabs(Cj[1]), std::fabs(Cj[2])));
+    if (flag == 1) return std::fmax(Cj[0], std::fmax(Cj[1], Cj[2]));
+    return std::fmin(Cj[0], std::fmin(Cj[1], Cj[2]));
+  }
+
+  // Latitude amplitudes (u, v, h) at one latitude (getPsi 180-208,
+  // getAmplitudes 213-258).
+  void amplitudes(double lat, int flag, double &ut, double &vt, double &ht) const {
+    const double C = phase_speed(flag), o2 = 2.0 * omega;
+    const double a3 = sigma * (sigma + 1) * (sigma + 2), a4 = a3 * (sigma + 3), a5 = a4 * (sigma + 4);
+    const double s = std::sin(lat), c = std::cos(lat), t = std::tan(lat);
+    const double s2 = s * s, s4 = s2 * s2;
+    const double C5 = (4.0 * a5 * s4 - 20.0 * a4 * s2 + 15.0 * a3) * s / 15.0;
+    const double C5p = (4.0 * a5 * s4 - 12.0 * a4 * s2 + 3.0 * a3) * c / 3.0;
+    const double amp = 1.0e-8, cs = std::pow(c, sigma);
+    const double psi = amp * cs * C5, dpsi = amp * cs * (-sigma * t * C5 + C5p);
+    if (flag == 0) {
+      const double Kp = (g * H0 + (a * a) * (C * C) * (c * c)) / (C * c);
+      const double Km = (g * H0 - (a * a) * (C * C) * (c * c)) / (C * c);
+      vt = std::sqrt(o2 * std::fabs(Km) / (c * c)) * psi;
+      ht = std::sqrt(o2 * std::fabs(Km) * (a * a) * (H0 * H0)) / Km * (dpsi + t * (0.5 * Kp / Km - o2 / C) * psi);
+    } else {
+      const double W1 = ((k * C) * (k * C) - (o2 * o2) * s2) / (C * c);
+      const double W2 = ((k * C) * (k * C) - (o2 * o2) * s2) / (o2 * (c * c));
+      vt = std::sqrt(o2 * std::fabs(W1) * g * H0 / (c * c)) / W1 * (dpsi + t * (0.5 - o2 / W2 + o2 / C) * psi);
+      ht = std::sqrt(o2 * std::fabs(W1) * (a * a) * H0 / g) * psi;
+    }
+    ut = (o2 * s / C) * vt + (g / a / c / C) * ht;
+    vt = k * vt;
+  }
+};
+
+}  // namespace
+
+// shallow_water_waves_test_mod.F90:96-136 at time 0: h = hTilde cos(k lon),
+// u = uTilde cos(k lon_half), v = vTilde cos(k lon - pi/2), gd = g h + 5e4,
+// ghs = 0.  As in the reference (getFields gets only the full latitudes for
+// its amplitudes, 261-305), v on half row j uses the amplitude of full row j.
+// u on the two pole rows, |u| < 1e-140 in the reference, is stored as 0.
+// flag: 0 Rossby (the reference's choice), 1 eastward / -1 westward
+// inertia-gravity wave.
+extern "C" int gd_ic_shallow_water_waves(int32_t N, int32_t M, int32_t flag, double *u, double *v,
+                                         double *gdo, double *ghs, int64_t ld, double *phase_speed) {
+  using namespace gd;
+  if (N < 8 || M < 5 || ld < N || flag < -1 || flag > 1) return GD_ERR_INVALID_ARG;
+  Geometry g;
+  if (!build_geometry(N, M, false, nullptr, 0, g).empty()) return GD_ERR_INVALID_ARG;
+  const ShamirPaldor sp;
+  std::vector<double> ut(M), vt(M), ht(M);
+  for (int j = 0; j < M; ++j) sp.amplitudes(g.full_lat[j], flag, ut[j], vt[j], ht[j]);
+  const double k = sp.k;
+  for (int j = 0; j < M; ++j)
+    for (int i = 0; i < N; ++i) {
+      const double h = ht[j] * std::cos(k * g.full_lon[i]);
+      if (gdo) gdo[(size_t)j * ld + i] = sp.g * h + 5.0e4;
+      if (ghs) ghs[(size_t)j * ld + i] = 0.0;
+      if (u) u[(size_t)j * ld + i] = (j == 0 || j == M - 1) ? 0.0 : ut[j] * std::cos(k * g.half_lon[i]);
+    }
+  if (v)
+    for (int j = 0; j < M - 1; ++j)
+      for (int i = 0; i < N; ++i) v[(size_t)j * ld + i] = vt[j] * std::cos(k * g.full_lon[i] - 0.5 * sp.pi);
+  if (phase_speed) *phase_speed = sp.phase_speed(flag);
   return GD_OK;
 }
 

@@ -7,8 +7,9 @@
 !   dycore_test.exe <namelist> --schedule  print the output files the run would
 !                                          write (time manager only, no GPU)
 !
-! Test cases: rossby_haurwitz_wave (&rossby_haurwitz_wave_test_params R, omg,
-! gd0), mountain_zonal_flow, and balanced_sh (&balanced_sh_params): a seeded
+! Test cases: the reference's steady_geostrophic_flow, rossby_haurwitz_wave
+! (&rossby_haurwitz_wave_test_params R, omg, gd0), mountain_zonal_flow,
+! jet_zonal_flow and shallow_water_waves, and balanced_sh (&balanced_sh_params): a seeded
 ! geostrophically balanced spherical-harmonic state, optionally with Gaussian
 ! mountains; ensemble member m adds the perturbation seed pert_seed + m - 1.
 program dycore_test
@@ -31,6 +32,7 @@ program dycore_test
   integer(8) :: seed = 1, mountain_seed = 0, pert_seed = 0
   integer :: nmax = 20, n_mountains = 0, pert_nmax = 0
   real(r8) :: slope = -3.0_r8, rms_height = 150.0_r8, mean_depth = 8000.0_r8, pert_rms = 0.0_r8
+  real(r8) :: phase_speed
   type(gd_ic_params) icp
 
   namelist /rossby_haurwitz_wave_test_params/ R, omg, gd0
@@ -72,6 +74,31 @@ contains
     call dycore_restart()
   else
     select case (test_case)
+    case ('steady_geostrophic_flow')
+      write(6, *) '[Notice]: Use steady geostrophic flow initial condition.'
+      do m = 1, members
+        rc = gd_ic_steady_geostrophic(int(num_lon, c_int32_t), int(num_lat, c_int32_t), &
+                                      state_u(1, 1, m), state_v(1, 1, m), state_gd(1, 1, m), &
+                                      static_ghs(1, 1, m), ld)
+        call check(rc)
+      end do
+    case ('jet_zonal_flow')
+      write(6, *) '[Notice]: Use jet zonal flow initial condition.'
+      do m = 1, members
+        rc = gd_ic_jet_zonal_flow(int(num_lon, c_int32_t), int(num_lat, c_int32_t), 1_c_int32_t, &
+                                  state_u(1, 1, m), state_v(1, 1, m), state_gd(1, 1, m), &
+                                  static_ghs(1, 1, m), ld)
+        call check(rc)
+      end do
+    case ('shallow_water_waves')
+      write(6, *) '[Notice]: Use shallow water waves initial condition.'
+      do m = 1, members
+        rc = gd_ic_shallow_water_waves(int(num_lon, c_int32_t), int(num_lat, c_int32_t), 0_c_int32_t, &
+                                       state_u(1, 1, m), state_v(1, 1, m), state_gd(1, 1, m), &
+                                       static_ghs(1, 1, m), ld, phase_speed)
+        call check(rc)
+      end do
+      write(6, '("[Notice]: Phase speed is ", ES27.20)') phase_speed
     case ('rossby_haurwitz_wave')
       write(6, *) '[Notice]: Use Rossby-Haurwitz wave initial condition.'
       call read_group(1)

@@ -146,6 +146,31 @@ module gd_capi_mod
       integer(c_int64_t), value :: ld
     end function gd_ic_mountain_zonal_flow
 
+    integer(c_int) function gd_ic_steady_geostrophic(num_lon, num_lat, u, v, gd, ghs, ld) &
+        bind(C, name='gd_ic_steady_geostrophic')
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: num_lon, num_lat
+      real(c_double), intent(out) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+    end function gd_ic_steady_geostrophic
+
+    integer(c_int) function gd_ic_jet_zonal_flow(num_lon, num_lat, perturb, u, v, gd, ghs, ld) &
+        bind(C, name='gd_ic_jet_zonal_flow')
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: num_lon, num_lat, perturb
+      real(c_double), intent(out) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+    end function gd_ic_jet_zonal_flow
+
+    integer(c_int) function gd_ic_shallow_water_waves(num_lon, num_lat, flag, u, v, gd, ghs, ld, &
+                                                      phase_speed) bind(C, name='gd_ic_shallow_water_waves')
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: num_lon, num_lat, flag
+      real(c_double), intent(out) :: u(*), v(*), gd(*), ghs(*)
+      integer(c_int64_t), value :: ld
+      real(c_double), intent(out) :: phase_speed
+    end function gd_ic_shallow_water_waves
+
     integer(c_int) function gd_cutoff_from_lat0(num_lon, num_lat, lat0_deg, cutoff, cap) &
         bind(C, name='gd_cutoff_from_lat0')
       import :: c_int, c_int32_t, c_double

@@ -184,6 +184,25 @@ int gd_ic_rossby_haurwitz(int32_t num_lon, int32_t num_lat, double R, double omg
 int gd_ic_mountain_zonal_flow(int32_t num_lon, int32_t num_lat, double *u, double *v,
                               double *gd, double *ghs, int64_t ld);
 
+/* steady_geostrophic_flow_test_mod.F90:20-62 (Williamson test 2, alpha = 0). */
+int gd_ic_steady_geostrophic(int32_t num_lon, int32_t num_lat, double *u, double *v, double *gd,
+                             double *ghs, int64_t ld);
+
+/* jet_zonal_flow_test_mod.F90:29-105 (Galewsky et al. 2004): the balanced
+ * jet, plus the Gaussian height bump when perturb != 0 (the reference always
+ * adds it).  The balance integral is evaluated to round-off (the reference
+ * uses QUADPACK qags with epsrel 1e-3). */
+int gd_ic_jet_zonal_flow(int32_t num_lon, int32_t num_lat, int32_t perturb, double *u,
+                         double *v, double *gd, double *ghs, int64_t ld);
+
+/* shallow_water_waves_test_mod.F90:96-305 (Shamir & Paldor 2016 analytic
+ * waves, (n, k) = (5, 10)) at time 0.  flag: 0 Rossby wave (the reference's
+ * choice), 1 / -1 eastward / westward inertia-gravity wave.  phase_speed (may
+ * be NULL) receives C in rad/s. */
+int gd_ic_shallow_water_waves(int32_t num_lon, int32_t num_lat, int32_t flag, double *u,
+                              double *v, double *gd, double *ghs, int64_t ld,
+                              double *phase_speed);
+
 /* Polar filter table whose kept wavenumber falls toward the poles as
  * cos(lat)/cos(lat0): entry k (k = 1.. from each pole, the layout of
  * zonal_tend_filter_cutoff_wavenumber, filter_mod.F90:44-99) drives full row

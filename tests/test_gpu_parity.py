@@ -165,3 +165,31 @@ def test_isp_steps_vs_oracle(gd, oracle_mod, precision):
     assert rel_err(v, o.state(o.old, oracle_mod.V)) < 100 * tol
     assert np.all(np.abs(beta[:, 0] - ob) <= 100 * tol * np.abs(ob))
     d.close()
+
+
+@pytest.mark.parametrize("case", ["steady_geostrophic", "jet_zonal_flow", "shallow_water_waves",
+                                  "rossby_haurwitz", "mountain_zonal_flow"])
+def test_reference_cases_vs_oracle(gd, oracle_mod, case):
+    """Each of the reference's test cases stepped on the device and by the
+    oracle from the same initial condition."""
+    N, M, dt = 144, 73, 600.0
+    ic = {"steady_geostrophic": lambda: gd.ic_steady_geostrophic(N, M),
+          "jet_zonal_flow": lambda: gd.ic_jet_zonal_flow(N, M),
+          "shallow_water_waves": lambda: gd.ic_shallow_water_waves(N, M)[:4],
+          "rossby_haurwitz": lambda: gd.ic_rossby_haurwitz(N, M),
+          "mountain_zonal_flow": lambda: gd.ic_mountain_zonal_flow(N, M)}[case]()
+    kw = dict(split_scheme="csp2", subcycles=4, use_diffusion=True, diffusion_coef=1e5)
+    cut = gd.cutoff_from_lat0(N, M, 70.0)
+    o = oracle_mod.Oracle(N, M, dt, cutoff=cut, **kw)
+    o.set_initial_state(*ic)
+    o.start()
+    with gd.Dycore(N, M, dt, zonal_tend_filter_cutoff_wavenumber=cut, **kw) as d:
+        d.set_state(*ic)
+        om, oe, ob, _ = o.run(4)
+        mass, energy, beta, _ = d.step(4)
+        u, v, h = d.get_state()
+    assert rel_err(h, o.state(o.old, oracle_mod.GD)) < 1e-11
+    assert rel_err(u, o.state(o.old, oracle_mod.U)) < 1e-10
+    assert rel_err(v, o.state(o.old, oracle_mod.V)) < 1e-9
+    assert np.all(np.abs(mass[:, 0] - om) <= 1e-13 * np.abs(om))
+    assert np.all(np.abs(energy[:, 0] - oe) <= 1e-12 * np.abs(oe))
