@@ -94,6 +94,22 @@ def run_gpu(gd, prec, ic, steps, warmup, repeats):
                 tend_us=tend_us)
 
 
+def run_batched(gd, prec, ic, members, steps):
+    """Aggregate model-days/s of `members` runs of the workload batched in one
+    handle (members stepped together by the captured step graph)."""
+    d, _ = make_dycore(gd, prec, members=members)
+    for m in range(members):
+        d.set_state(*ic, member=m)
+    d.step(10)
+    for m in range(members):
+        d.set_state(*ic, member=m)
+    t0 = time.perf_counter()
+    d.step(steps)
+    el = time.perf_counter() - t0
+    d.close()
+    return members * steps * WORKLOAD["time_step_size"] / 86400.0 / el, el
+
+
 def pmc_probe(gd, ic, nsteps):
     """Workload of the hardware-counter runs: a plain device-to-device copy of
     256 MiB through the HIP runtime (its FETCH_SIZE / WRITE_SIZE per dispatch
@@ -118,6 +134,130 @@ def pmc_probe(gd, ic, nsteps):
     d.step(nsteps)
     d.close()
     print(f"pmc probe: {nsteps} steps done", flush=True)
+
+
+# ---- the other workloads (run with --workload; not the default line) -------
+
+def _cfg(gd, N, M, dt, coef, prec, members=1):
+    cut = gd.cutoff_from_lat0(N, M, FILTER_LAT0)
+    kw = dict(WORKLOAD, num_lon=N, num_lat=M, time_step_size=dt, diffusion_coef=coef)
+    return gd.Dycore(zonal_tend_filter_cutoff_wavenumber=cut, precision=prec, members=members, **kw)
+
+
+def _time_steps(d, steps, warmup):
+    d.step(warmup)
+    t0 = time.perf_counter()
+    d.step(steps)
+    return time.perf_counter() - t0
+
+
+def other_workload(gd, args):
+    base = {"n_gpus": 1, "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None,
+            "dtype": "f32/f64", "roofline": None, "cpu_baseline": None}
+    if args.workload == "ensemble":
+        # 64 members at 360x181, dt 240 s, 5 model days (1800 steps); members
+        # share the topography-free balanced state plus their own rms-5 m
+        # perturbation
+        N, M, dt, mem = 360, 181, 240.0, 64
+        steps = min(args.steps, 1800)
+        ics = [gd.ic_balanced(N, M, seed=20243, nmax=20, pert_seed=5000 + k, pert_nmax=20, pert_rms=5.0)
+               for k in range(mem)]
+        out = {}
+        for prec in ("mixed", "fp64"):
+            d = _cfg(gd, N, M, dt, 1.0e5, prec, members=mem)
+            for k, ic in enumerate(ics):
+                d.set_state(*ic, member=k)
+            el = _time_steps(d, steps, args.warmup)
+            d.close()
+            out[prec] = mem * steps * dt / 86400.0 / el
+        return dict(base, metric="member-days/sec, 64-member ensemble at 360x181 (mixed vs FP64)",
+                    value=round(out["mixed"], 3), unit="member-days/s", steps=steps,
+                    ms_per_step=round(mem * steps * dt / 86400.0 / out["mixed"] / steps * 1e3, 4),
+                    scaling="weak", data="synthetic (seeded balanced state + per-member perturbations)",
+                    config={"workload": "ensemble 64 x 360x181, dt=240 s, csp2 x6, diffusion 1e5, "
+                                        f"filter cos(lat)/cos({FILTER_LAT0:g})",
+                            "fp64_member_days_per_s": round(out["fp64"], 3),
+                            "mixed_over_fp64": round(out["mixed"] / out["fp64"], 3)})
+    if args.workload == "1440":
+        N, M, dt = 1440, 721, 60.0
+        steps = min(args.steps, 2880)
+        ic = gd.ic_balanced(N, M, seed=20242, nmax=80)
+        out = {}
+        for prec in ("mixed", "fp64"):
+            d = _cfg(gd, N, M, dt, 6.25e3, prec)
+            d.set_state(*ic)
+            el = _time_steps(d, steps, args.warmup)
+            d.close()
+            out[prec] = steps * dt / 86400.0 / el
+        return dict(base, metric="model-days/sec at 1440x721 (mixed vs FP64)",
+                    value=round(out["mixed"], 4), unit="model-days/s", steps=steps,
+                    ms_per_step=round(dt / 86400.0 / out["mixed"] * 1e3, 4), scaling="weak",
+                    data="synthetic (seeded balanced state, n <= 80)",
+                    config={"workload": "1440x721 (0.25 deg), dt=60 s, csp2 x6, diffusion 6.25e3, "
+                                        f"filter cos(lat)/cos({FILTER_LAT0:g})",
+                            "fp64_model_days_per_s": round(out["fp64"], 4),
+                            "mixed_over_fp64": round(out["mixed"] / out["fp64"], 3)})
+    return serving_workload(gd, args, base)
+
+
+# (num_lon, num_lat, dt, diffusion): diffusion 1e5 at 1 deg and above, scaled
+# with the grid spacing squared below (4e5 at 2 deg is unstable next to the
+# poles: the oracle reaches NaN at step 124)
+SERVE_CLASSES = [(180, 91, 480.0, 1.0e5), (360, 181, 240.0, 1.0e5), (720, 361, 120.0, 2.5e4)]
+
+
+def serving_workload(gd, args, base):
+    """Poisson arrivals of runs with resolution uniform over 180x91, 360x181,
+    720x361 (dt 480/240/120 s, diffusion of SERVE_CLASSES)
+    and length uniform over 1..10 model days, seeded balanced states.  The
+    arrival rate is 0.8 / (mean single-run service time), the service times
+    measured alone first on the same GPU."""
+    rng = np.random.default_rng(4242)
+    nreq = max(4, args.serve_requests)
+    # single-run ms/step of each class (mixed), for the arrival rate
+    ms = {}
+    for (N, M, dt, coef) in SERVE_CLASSES:
+        d = _cfg(gd, N, M, dt, coef, "mixed")
+        d.set_state(*gd.ic_balanced(N, M, seed=1, nmax=20))
+        ms[(N, M)] = _time_steps(d, 200, 20) / 200 * 1e3
+        d.close()
+    reqs = []
+    t = 0.0
+    cls = rng.integers(0, 3, nreq)
+    days = rng.integers(1, 11, nreq)
+    service = [days[k] * 86400 / SERVE_CLASSES[cls[k]][2] * ms[SERVE_CLASSES[cls[k]][:2]] / 1e3
+               for k in range(nreq)]
+    rate = 0.8 / float(np.mean(service))
+    for k in range(nreq):
+        N, M, dt, coef = SERVE_CLASSES[cls[k]]
+        reqs.append(dict(num_lon=N, num_lat=M, time_step_size=dt, diffusion_coef=coef,
+                         nsteps=int(days[k] * 86400 / dt), arrival_s=t,
+                         state=gd.ic_balanced(N, M, seed=9000 + k, nmax=20)))
+        t += float(rng.exponential(1.0 / rate))
+    out = {}
+    for prec in ("mixed", "fp64"):
+        res, wall = gd.serve(reqs, precision=prec, filter_lat0=FILTER_LAT0, max_slots=16,
+                             chunk_steps=60, split_scheme="csp2", subcycles=6)
+        assert all(r["status"] == 0 for r in res), [r["status"] for r in res]
+        ttr = np.array([r["finish_s"] - r["arrival_s"] for r in res])
+        total_days = float(np.sum(days))
+        out[prec] = dict(value=total_days / wall, wall=wall, p50=float(np.percentile(ttr, 50)),
+                         p99=float(np.percentile(ttr, 99)))
+    m = out["mixed"]
+    return dict(base, metric="aggregate model-days/sec of a Poisson stream of runs "
+                             "(180x91/360x181/720x361, 1-10 days); p50/p99 time-to-result",
+                value=round(m["value"], 3), unit="model-days/s", steps=None, ms_per_step=None,
+                scaling="weak", data="synthetic (seeded balanced states)",
+                config={"workload": f"serving stream of {nreq} runs, rate {rate:.3f}/s "
+                                    "(0.8 of the mean single-run service time)",
+                        "requests": nreq, "model_days": int(np.sum(days)),
+                        "single_run_ms_per_step": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in ms.items()},
+                        "fp64_model_days_per_s": round(out["fp64"]["value"], 3),
+                        "mixed_over_fp64": round(m["value"] / out["fp64"]["value"], 3)},
+                latency_s={"p50": round(m["p50"], 4), "p99": round(m["p99"], 4),
+                           "fp64_p50": round(out["fp64"]["p50"], 4),
+                           "fp64_p99": round(out["fp64"]["p99"], 4),
+                           "stream_wall_s": round(m["wall"], 3)})
 
 
 def cpu_baseline(ic, nsteps):
@@ -152,6 +292,13 @@ def main():
     ap.add_argument("--repeats", type=int, default=3, help="full 10-day runs for p50/p99 latency")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-members", type=int, default=8,
+                    help="members of the batched mixed-precision form timed beside the single run")
+    ap.add_argument("--batch-steps", type=int, default=720)
+    ap.add_argument("--workload", default="cfg2", choices=["cfg2", "ensemble", "1440", "serving"],
+                    help="cfg2: the 720x361 single run (default); the others time the "
+                         "ensemble (64 x 360x181), 1440x721 and serving-stream workloads")
+    ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--pmc-probe", type=int, default=0, metavar="K",
                     help="counter runs: one 256 MiB device copy (calibration of the byte "
                          "counters) then K steps of the mixed workload, no timing output")
@@ -167,9 +314,16 @@ def main():
     if args.pmc_probe:
         pmc_probe(gd, ic, args.pmc_probe)
         return
+    if args.workload != "cfg2":
+        print(json.dumps(other_workload(gd, args)))
+        return
     res = {}
     for prec in ("mixed", "fp64"):
         res[prec] = run_gpu(gd, prec, ic, args.steps, args.warmup, args.repeats)
+    batched = None
+    if args.batch_members > 1:
+        bv, bel = run_batched(gd, "mixed", ic, args.batch_members, args.batch_steps)
+        batched = (bv, bel)
     days = args.steps * dt / 86400.0
     val = {p: days / r["elapsed"] for p, r in res.items()}
     mixed = res["mixed"]
@@ -216,6 +370,10 @@ def main():
                    "precision": "mixed", "fp64_model_days_per_s": round(val["fp64"], 4),
                    "mixed_over_fp64": round(val["mixed"] / val["fp64"], 3),
                    "filtered_rows_per_hemisphere": len(mixed["cut"]),
+                   "batched_mixed": None if batched is None else {
+                       "members": args.batch_members, "steps": args.batch_steps,
+                       "aggregate_model_days_per_s": round(batched[0], 4),
+                       "over_fp64_single_run": round(batched[0] / val["fp64"], 3)},
                    "parallelism": "single GPU, single run"},
         "latency_s": {"runs": int(len(mixed["lat"])), "p50": float(np.percentile(lat, 50)),
                       "p99": float(np.percentile(lat, 99)),

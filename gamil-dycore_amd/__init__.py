@@ -51,6 +51,30 @@ class IOInfo(C.Structure):
     ]
 
 
+class ServeRequest(C.Structure):
+    _fields_ = [
+        ("num_lon", C.c_int32), ("num_lat", C.c_int32), ("time_step_size", C.c_double),
+        ("diffusion_coef", C.c_double), ("nsteps", C.c_int32), ("arrival_s", C.c_double),
+        ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)), ("gd", C.POINTER(C.c_double)),
+        ("ghs", C.POINTER(C.c_double)),
+    ]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("arrival_s", C.c_double), ("start_s", C.c_double), ("finish_s", C.c_double),
+        ("total_mass", C.c_double), ("total_energy", C.c_double), ("state_hash", C.c_uint64),
+        ("status", C.c_int32), ("slot", C.c_int32),
+    ]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [
+        ("base", C.c_void_p), ("filter_lat0", C.c_double), ("hbm_budget_bytes", C.c_int64),
+        ("max_slots", C.c_int32), ("chunk_steps", C.c_int32),
+    ]
+
+
 class ICParams(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64), ("nmax", C.c_int32), ("slope", C.c_double),
@@ -68,7 +92,8 @@ EXPORTS = (
     "gd_ic_rossby_haurwitz", "gd_ic_mountain_zonal_flow", "gd_profile_kernels",
     "gd_device_bytes", "gd_cutoff_from_lat0", "gd_time_kernel", "gd_history_write",
     "gd_restart_write", "gd_restart_read", "gd_io_last_error", "gd_ic_steady_geostrophic",
-    "gd_ic_jet_zonal_flow", "gd_ic_shallow_water_waves",
+    "gd_ic_jet_zonal_flow", "gd_ic_shallow_water_waves", "gd_serve", "gd_serve_last_error",
+    "gd_set_active_members", "gd_copy_member",
 )
 
 _lib = None
@@ -121,6 +146,12 @@ def lib():
                                   C.c_int32, dp]
     L.gd_io_last_error.restype = C.c_char_p
     L.gd_io_last_error.argtypes = []
+    L.gd_set_active_members.argtypes = [vp, C.c_int32]
+    L.gd_copy_member.argtypes = [vp, C.c_int32, C.c_int32]
+    L.gd_serve.argtypes = [C.POINTER(ServeConfig), C.POINTER(ServeRequest), C.c_int32,
+                           C.POINTER(ServeResult), dp]
+    L.gd_serve_last_error.restype = C.c_char_p
+    L.gd_serve_last_error.argtypes = []
     _lib = L
     return L
 
@@ -265,6 +296,44 @@ def read_restart(path, num_lon, num_lat):
 
 
 # ---------------------------------------------------------------------------
+# serving: continuous batching of independent runs (gd_serve)
+# ---------------------------------------------------------------------------
+
+def serve(requests, precision="mixed", split_scheme="csp2", subcycles=6, qcon_modified=True,
+          uv_adv_scheme="center_diff", use_diffusion=True, diffusion_order=2,
+          use_zonal_tend_filter=True, filter_lat0=75.0, hbm_budget_bytes=64 << 30, max_slots=64,
+          chunk_steps=60, device=0):
+    """Serve a stream of runs.  Each request is a dict with num_lon, num_lat,
+    time_step_size, nsteps, arrival_s, diffusion_coef and state = (u, v, gd,
+    ghs) dense arrays.  Returns (results, wall_seconds); results[k] is a dict
+    for requests[k] (arrival_s, start_s, finish_s, total_mass, total_energy,
+    state_hash, status, slot)."""
+    L = lib()
+    keep = []
+    reqs = (ServeRequest * max(1, len(requests)))()
+    for k, q in enumerate(requests):
+        N, M = int(q["num_lon"]), int(q["num_lat"])
+        u, v, g, hs = (_dense(a, r, N) for a, r in zip(q["state"], (M, M - 1, M, M)))
+        keep += [u, v, g, hs]
+        reqs[k] = ServeRequest(N, M, float(q["time_step_size"]), float(q.get("diffusion_coef", 0.0)),
+                               int(q["nsteps"]), float(q["arrival_s"]), _dp(u), _dp(v), _dp(g), _dp(hs))
+    cut = np.zeros(1, np.int32)
+    base = _Config(8, 5, 1.0, SPLIT[split_scheme], int(subcycles), int(bool(qcon_modified)),
+                   ADV[uv_adv_scheme], 0.0, 0.5, int(bool(use_diffusion)), int(diffusion_order), 0.0,
+                   int(bool(use_zonal_tend_filter)), 0, cut.ctypes.data_as(C.POINTER(C.c_int32)),
+                   PRECISION[precision], int(device))
+    cfg = ServeConfig(C.cast(C.pointer(base), C.c_void_p), float(filter_lat0), int(hbm_budget_bytes),
+                      int(max_slots), int(chunk_steps))
+    res = (ServeResult * max(1, len(requests)))()
+    wall = C.c_double(0.0)
+    code = L.gd_serve(C.byref(cfg), reqs, len(requests), res, C.byref(wall))
+    if code != GD_OK:
+        raise DycoreError(code, L.gd_serve_last_error().decode(errors="replace"))
+    out = [{f: getattr(res[k], f) for f, _ in ServeResult._fields_} for k in range(len(requests))]
+    return out, wall.value
+
+
+# ---------------------------------------------------------------------------
 # the dycore
 # ---------------------------------------------------------------------------
 
@@ -338,6 +407,13 @@ class Dycore:
             return mass, energy, beta, ns.value
         _check(code, self.h)
         return mass, energy, beta, ns.value
+
+    def set_active_members(self, n_active):
+        """Step only members 0..n_active-1 from now on."""
+        _check(lib().gd_set_active_members(self.h, int(n_active)), self.h)
+
+    def copy_member(self, dst, src):
+        _check(lib().gd_copy_member(self.h, int(dst), int(src)), self.h)
 
     # -- operator seams ----------------------------------------------------
     def space_operators(self, pass_="all", member=0):

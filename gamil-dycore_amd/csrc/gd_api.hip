@@ -36,6 +36,7 @@ struct gd_handle {
   std::vector<int32_t> cutoff;
   Geometry g;
   int nmem = 1, prec = 0, N = 0, M = 0, P = 0, cts = 8;
+  int nact = 1;                  // members 0..nact-1 are stepped
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
@@ -53,7 +54,7 @@ struct gd_handle {
   int32_t *dstep = nullptr;
   double *ddiag = nullptr;  // [nmem][cap][3]
   int diag_cap = 512;
-  hipGraphExec_t graph = nullptr;
+  std::map<int, hipGraphExec_t> graphs;  // captured step per active-member count
   bool graph_failed = false;
   std::string err;
   std::vector<ProfEntry> *prof = nullptr;
@@ -111,15 +112,15 @@ Args base_args(const gd_handle *h) {
   a.N = h->N;
   a.M = h->M;
   a.pitch = h->P;
-  a.nmem = h->nmem;
+  a.nmem = h->nact;
   a.m0 = 0;
   a.nseg = (h->N + kSeg - 1) / kSeg;
-  a.ntiles = h->nmem * h->M * a.nseg;  // (member, row, segment) tiles
+  a.ntiles = h->nact * h->M * a.nseg;  // (member, row, segment) tiles
   a.nslots = (int)h->slots.size();
   // k_update: one row per workgroup for a single run; ensembles share about
   // 512 workgroups (each forms its member's beta from the partials)
-  a.nmem_launch = h->nmem;
-  a.upd_wpm = std::max(1, std::min(h->M, 512 / h->nmem));
+  a.nmem_launch = h->nact;
+  a.upd_wpm = std::max(1, std::min(h->M, 512 / h->nact));
   a.upwind = h->cfg.uv_adv_scheme == GD_ADV_UPWIND;
   a.beta_lon = h->cfg.uv_adv_upwind_lon_beta;
   a.beta_lat = h->cfg.uv_adv_upwind_lat_beta;
@@ -149,7 +150,8 @@ hipError_t run(gd_handle *h, const char *family, F &&f) {
 
 hipError_t enqueue_filter(gd_handle *h, Args a) {
   if (h->frows.empty()) return hipSuccess;
-  a.ntiles = (int)h->frows.size();
+  // member-major: the first nact members' rows
+  a.ntiles = (int)(h->frows.size() / h->nmem) * h->nact;
   return run(h, "filter", [&] { return launch_filter(a, h->prec, h->stream); });
 }
 
@@ -161,7 +163,7 @@ hipError_t enqueue_call(gd_handle *h, double dt, int pass, int s0, int s_out, in
   a.dt = dt;
   a.hdt = dt * 0.5;
   hipError_t e;
-  a.ntiles = h->nmem * h->M * a.nseg;  // (member, row, segment) tiles
+  a.ntiles = h->nact * h->M * a.nseg;  // (member, row, segment) tiles
   // first predictor: T = L(S0), S1 = S0 + dt/2 T
   a.phase = PH_A; a.s_in = s0; a.s0 = s0; a.s_out = SP1;
   e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, s0 == SX, h->stream); });
@@ -308,39 +310,45 @@ hipError_t enqueue_step(gd_handle *h) {
   }
   if (c.use_diffusion) {
     if ((e = enqueue_diffusion(h, last, false, dt)) != hipSuccess) return e;
-    return enqueue_finish(h, 0, SX, 0, h->nmem);
+    return enqueue_finish(h, 0, SX, 0, h->nact);
   }
-  return enqueue_finish(h, 1, last, 0, h->nmem);
+  return enqueue_finish(h, 1, last, 0, h->nact);
 }
 
-int build_step_graph(gd_handle *h) {
-  if (h->graph || h->graph_failed) return GD_OK;
+// The captured step of the current active-member count (nullptr: launch
+// eagerly).
+hipGraphExec_t step_graph(gd_handle *h) {
+  if (h->graph_failed) return nullptr;
+  auto it = h->graphs.find(h->nact);
+  if (it != h->graphs.end()) return it->second;
   // GD_NO_GRAPH=1 launches every kernel eagerly (tracing tools that do not
   // follow graph launches)
   const char *ng = std::getenv("GD_NO_GRAPH");
   if (ng && ng[0] == '1') {
     h->graph_failed = true;
-    return GD_OK;
+    return nullptr;
   }
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) { h->graph_failed = true; return GD_OK; }
+  if (e != hipSuccess) { h->graph_failed = true; return nullptr; }
   hipError_t e2 = enqueue_step(h);
   e = hipStreamEndCapture(h->stream, &g);
   if (e2 != hipSuccess || e != hipSuccess) {
     if (g) hipGraphDestroy(g);
     hipGetLastError();
     h->graph_failed = true;
-    return GD_OK;
+    return nullptr;
   }
-  e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) {
-    h->graph = nullptr;
     h->graph_failed = true;
     hipGetLastError();
+    return nullptr;
   }
-  return GD_OK;
+  h->graphs[h->nact] = x;
+  return x;
 }
 
 // Staging slots, filter workgroups and the per-row coefficient table.
@@ -484,6 +492,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->cutoff.assign(cfg->cutoff, cfg->cutoff + std::max(0, cfg->n_cutoff));
   h->cfg.cutoff = h->cutoff.data();
   h->nmem = n_members;
+  h->nact = n_members;
   h->prec = cfg->precision == GD_PREC_FP64 ? 0 : 1;
   h->cts = h->prec == 0 ? 8 : 4;
   h->N = cfg->num_lon;
@@ -579,7 +588,7 @@ int gd_destroy(gd_handle *h) {
   if (!h) return GD_OK;
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
-  if (h->graph) hipGraphExecDestroy(h->graph);
+  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
   for (void *p : h->allocs) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -629,15 +638,15 @@ int gd_step(gd_handle *h, int32_t nsteps, double *mass, double *energy, double *
   if (!h || nsteps < 0) return GD_ERR_INVALID_ARG;
   hipSetDevice(h->device);
   if (nan_step) *nan_step = 0;
-  if (!h->prof) build_step_graph(h);
+  hipGraphExec_t graph = h->prof ? nullptr : step_graph(h);
   std::vector<double> buf((size_t)h->nmem * h->diag_cap * 3);
   int first_nan = 0;
   for (int done = 0; done < nsteps;) {
     const int n = std::min(h->diag_cap, nsteps - done);
     GD_HIP(h, hipMemsetAsync(h->dstep, 0, sizeof(int32_t) * h->nmem, h->stream));
     for (int k = 0; k < n; ++k) {
-      if (h->graph && !h->prof) {
-        GD_HIP(h, hipGraphLaunch(h->graph, h->stream));
+      if (graph) {
+        GD_HIP(h, hipGraphLaunch(graph, h->stream));
       } else {
         GD_HIP(h, enqueue_step(h));
       }
@@ -651,7 +660,7 @@ int gd_step(gd_handle *h, int32_t nsteps, double *mass, double *energy, double *
         if (mass) mass[o] = d[0];
         if (energy) energy[o] = d[1];
         if (beta) beta[o] = d[2];
-        if (!first_nan && (std::isnan(d[0]) || std::isnan(d[1]))) first_nan = done + k + 1;
+        if (m < h->nact && !first_nan && (std::isnan(d[0]) || std::isnan(d[1]))) first_nan = done + k + 1;
       }
     done += n;
   }
@@ -813,6 +822,25 @@ int gd_profile_kernels(gd_handle *h, int32_t nsteps, int32_t max_families, int32
 }
 
 int64_t gd_device_bytes(const gd_handle *h) { return h ? h->bytes : 0; }
+
+int gd_set_active_members(gd_handle *h, int32_t n_active) {
+  if (!h) return GD_ERR_INVALID_ARG;
+  if (n_active < 1 || n_active > h->nmem) return fail(h, GD_ERR_INVALID_ARG, "n_active out of range");
+  h->nact = n_active;
+  return GD_OK;
+}
+
+int gd_copy_member(gd_handle *h, int32_t dst, int32_t src) {
+  int rc = check_member(h, dst);
+  if (rc) return rc;
+  if ((rc = check_member(h, src))) return rc;
+  if (dst == src) return GD_OK;
+  hipSetDevice(h->device);
+  GD_HIP(h, hipMemcpyAsync(h->arena + (size_t)dst * h->mstride, h->arena + (size_t)src * h->mstride,
+                           h->mstride, hipMemcpyDeviceToDevice, h->stream));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  return GD_OK;
+}
 
 int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us) {
   if (!h || reps < 1 || !us || kernel < GD_TIME_TEND || kernel > GD_TIME_UPDATE) return GD_ERR_INVALID_ARG;

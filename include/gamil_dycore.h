@@ -110,6 +110,15 @@ int gd_diag(gd_handle *h, double *total_mass, double *total_energy);
 int gd_step(gd_handle *h, int32_t nsteps, double *total_mass, double *total_energy,
             double *beta, int32_t *nan_step);
 
+/* Ensemble bookkeeping (no counterpart in the reference, which holds one
+ * run): gd_step, gd_predict_correct and gd_ordinary_diffusion advance members
+ * 0..n_active-1 only (default: all); the other members keep their state.
+ * Their entries in gd_step's outputs are not defined.  gd_copy_member copies
+ * member src's complete device state over member dst, so a run continues bit
+ * for bit in another slot. */
+int gd_set_active_members(gd_handle *h, int32_t n_active);
+int gd_copy_member(gd_handle *h, int32_t dst, int32_t src);
+
 /* D2H copy of the prognostic state (u, v, gd; any pointer may be NULL) and
  * of the history diagnostics div and vor (diag_mod.F90:49-77, may be NULL). */
 int gd_get_state(gd_handle *h, int32_t member, double *u, double *v, double *gd,
@@ -268,6 +277,52 @@ int gd_restart_read(const char *path, int32_t num_lon, int32_t num_lat, double *
 
 /* Text of the last failed history/restart call on this thread. */
 const char *gd_io_last_error(void);
+
+/* ---- serving: continuous batching of independent runs ------------------ */
+/* No counterpart in the reference (a single-run program): the batching and
+ * serving layer over the step above.  Runs of one (resolution, time step,
+ * diffusion coefficient) class share a pool of device-resident member slots
+ * sized by the HBM budget; requests are admitted when they have arrived and a
+ * slot of their class is free; pools of different classes run concurrently
+ * on their own streams; a run gives the same bits alone and in any mix. */
+
+typedef struct gd_serve_request {
+  int32_t num_lon, num_lat;
+  double time_step_size;
+  double diffusion_coef;       /* used when base->use_diffusion is set          */
+  int32_t nsteps;              /* run length                                    */
+  double arrival_s;            /* arrival, seconds after the stream starts      */
+  const double *u, *v, *gd, *ghs;  /* initial state, dense rows (ld = num_lon)  */
+} gd_serve_request;
+
+typedef struct gd_serve_result {
+  double arrival_s;            /* copied from the request                        */
+  double start_s;              /* admitted into a slot                           */
+  double finish_s;             /* result ready (time-to-result = finish - arrival) */
+  double total_mass;           /* of the final state                             */
+  double total_energy;
+  uint64_t state_hash;         /* FNV-1a of the bytes of the final u, v, gd      */
+  int32_t status;              /* GD_OK, GD_ERR_NAN, ...                         */
+  int32_t slot;
+} gd_serve_result;
+
+typedef struct gd_serve_config {
+  const gd_config *base;       /* options shared by every run; num_lon, num_lat,
+                                  time_step_size, diffusion_coef and the filter
+                                  table are set per class                        */
+  double filter_lat0;          /* per-class filter table from cos(lat)/cos(lat0)
+                                  (gd_cutoff_from_lat0); 0: no filter rows       */
+  int64_t hbm_budget_bytes;    /* device memory for resident runs               */
+  int32_t max_slots;           /* members per pool at most                       */
+  int32_t chunk_steps;         /* steps between admission checks                 */
+} gd_serve_config;
+
+/* Serves the n requests (arrival times relative to the call) and returns when
+ * every run has finished; res[k] belongs to req[k].  wall_seconds (may be
+ * NULL) receives the duration of the stream. */
+int gd_serve(const gd_serve_config *cfg, const gd_serve_request *req, int32_t n,
+             gd_serve_result *res, double *wall_seconds);
+const char *gd_serve_last_error(void);
 
 /* ---- measurement -------------------------------------------------------- */
 

@@ -100,3 +100,18 @@ def test_cutoff_table_from_lat0(gd):
     assert all(1 <= c <= 359 for c in cut)
     lat = np.deg2rad(-90 + 0.5)
     assert cut[0] == int(np.floor(360 * np.cos(lat) / np.cos(np.deg2rad(75.0))))
+
+
+def test_serve_rejects_bad_requests(gd):
+    """Argument checks of gd_serve run before any device call."""
+    N, M = 72, 37
+    ic = gd.ic_balanced(N, M, seed=1, nmax=4)
+    ok = dict(num_lon=N, num_lat=M, time_step_size=600.0, nsteps=3, arrival_s=0.0, state=ic)
+    with pytest.raises(gd.DycoreError, match="bad request 0"):
+        gd.serve([dict(ok, time_step_size=0.0)])
+    with pytest.raises(gd.DycoreError, match="bad request 1"):
+        gd.serve([ok, dict(ok, arrival_s=-1.0)])
+    with pytest.raises(gd.DycoreError, match="max_slots"):
+        gd.serve([ok], max_slots=0)
+    res, wall = gd.serve([])
+    assert res == [] and wall >= 0

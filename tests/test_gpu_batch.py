@@ -150,3 +150,33 @@ def test_time_kernel_leaves_state_untouched(gd):
         mb, eb, _, _ = ref.step(2)
         np.testing.assert_array_equal(ma, mb)
         np.testing.assert_array_equal(ea, eb)
+
+
+def test_active_members_and_member_copy(gd):
+    """Only the active prefix is stepped; a copied member continues bit for bit
+    (the serving pools compact their runs this way)."""
+    N, M, dt = 72, 37, 900.0
+    cfg = _cfg(gd, N, M, subcycles=3)
+    ics = [gd.ic_balanced(N, M, seed=3, nmax=8, pert_seed=40 + k, pert_nmax=8, pert_rms=5.0)
+           for k in range(3)]
+    with gd.Dycore(N, M, dt, members=3, **cfg) as e, gd.Dycore(N, M, dt, **cfg) as one:
+        for k, ic in enumerate(ics):
+            e.set_state(*ic, member=k)
+        frozen = e.get_state(member=2)
+        e.set_active_members(2)
+        mass, _, _, _ = e.step(5)
+        for a, b in zip(e.get_state(member=2), frozen):
+            np.testing.assert_array_equal(a, b)
+        one.set_state(*ics[0])
+        m1, _, _, _ = one.step(5)
+        np.testing.assert_array_equal(mass[:, 0], m1[:, 0])
+        e.copy_member(2, 0)
+        e.set_active_members(3)
+        mass, _, _, _ = e.step(4)
+        m1, _, _, _ = one.step(4)
+        np.testing.assert_array_equal(mass[:, 2], m1[:, 0])
+        np.testing.assert_array_equal(mass[:, 0], m1[:, 0])
+        for a, b in zip(e.get_state(member=2), one.get_state()):
+            np.testing.assert_array_equal(a, b)
+        with pytest.raises(gd.DycoreError):
+            e.set_active_members(4)

@@ -167,19 +167,30 @@ def test_isp_steps_vs_oracle(gd, oracle_mod, precision):
     d.close()
 
 
-@pytest.mark.parametrize("case", ["steady_geostrophic", "jet_zonal_flow", "shallow_water_waves",
-                                  "rossby_haurwitz", "mountain_zonal_flow"])
-def test_reference_cases_vs_oracle(gd, oracle_mod, case):
+# The Rossby-Haurwitz wave and the Shamir-Paldor wave are pure zonal
+# wavenumber patterns: on every row the filter's inner product s1 = sum(tend *
+# weight) vanishes in exact arithmetic and is left as round-off of 1e-16 of
+# its terms, which still passes the reference's |s1| > 1e-16 test
+# (dycore_mod.F90:212-219), so the rescale s1/s2 is a ratio of round-off
+# residues that depends on the summation order (tests/test_oracle.py::
+# test_filter_rescale_is_ill_conditioned_for_pure_waves).  Those two cases are
+# compared without the filter, on a grid and time step whose pole rows are
+# stable without it (fast step 75 s, 2.5 deg).
+@pytest.mark.parametrize("case,use_filter", [("steady_geostrophic", True), ("jet_zonal_flow", True),
+                                             ("shallow_water_waves", False), ("rossby_haurwitz", False),
+                                             ("mountain_zonal_flow", True)])
+def test_reference_cases_vs_oracle(gd, oracle_mod, case, use_filter):
     """Each of the reference's test cases stepped on the device and by the
     oracle from the same initial condition."""
-    N, M, dt = 144, 73, 600.0
+    N, M, dt = (144, 73, 600.0) if use_filter else (72, 37, 300.0)
     ic = {"steady_geostrophic": lambda: gd.ic_steady_geostrophic(N, M),
           "jet_zonal_flow": lambda: gd.ic_jet_zonal_flow(N, M),
           "shallow_water_waves": lambda: gd.ic_shallow_water_waves(N, M)[:4],
           "rossby_haurwitz": lambda: gd.ic_rossby_haurwitz(N, M),
           "mountain_zonal_flow": lambda: gd.ic_mountain_zonal_flow(N, M)}[case]()
-    kw = dict(split_scheme="csp2", subcycles=4, use_diffusion=True, diffusion_coef=1e5)
-    cut = gd.cutoff_from_lat0(N, M, 70.0)
+    kw = dict(split_scheme="csp2", subcycles=4, use_diffusion=True, diffusion_coef=1e5,
+              use_zonal_tend_filter=use_filter)
+    cut = gd.cutoff_from_lat0(N, M, 70.0) if use_filter else []
     o = oracle_mod.Oracle(N, M, dt, cutoff=cut, **kw)
     o.set_initial_state(*ic)
     o.start()

@@ -371,3 +371,28 @@ def test_diffusion_fourth_order_eigenvalue(oracle_mod):
         inner = np.abs(lat) <= np.radians(60.0)
         errs.append(np.abs(delta[inner] - expect[inner]).max() / np.abs(expect).max())
     assert errs[0] < 2e-2 and errs[1] < errs[0] / 3.0
+
+
+def test_filter_rescale_is_ill_conditioned_for_pure_waves(oracle_mod):
+    """For a pure zonal-wavenumber state (Rossby-Haurwitz wave) the filter's
+    s1 = sum(tend * weight) on a row vanishes analytically and is only
+    round-off: sum|t w| / |sum t w| exceeds 1e12 on the polar rows, so the
+    reference's rescale tend * s1/s2 (dycore_mod.F90:212-219) multiplies the
+    filtered tendency by a ratio of round-off residues.  A balanced random
+    state is well conditioned."""
+    import __graft_entry__ as ge
+    gd = ge.load_package()
+    N, M = 144, 73
+
+    def kappa(ic):
+        o = oracle_mod.Oracle(N, M, 600.0, use_zonal_tend_filter=False)
+        o.set_initial_state(*ic)
+        o.start()
+        o.space_operators(o.old, 0, oracle_mod.ALL_PASS)
+        du = o.tend(0, oracle_mod.T_DU)
+        U = o.state(o.old, oracle_mod.IAP_U)
+        rows = list(range(1, 8)) + list(range(M - 8, M - 1))
+        return max(np.abs(du[j] * U[j]).sum() / abs((du[j] * U[j]).sum()) for j in rows)
+
+    assert kappa(gd.ic_rossby_haurwitz(N, M)) > 1e12
+    assert kappa(gd.ic_balanced(N, M, seed=3, nmax=10, n_mountains=2, mountain_seed=103)) < 1e5
