@@ -332,76 +332,12 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
 }
 
 // ---------------------------------------------------------------------------
-// beta of predict_correct (dycore_mod.F90:784-787), one workgroup per member
-// ---------------------------------------------------------------------------
-
-__global__ void __launch_bounds__(kBlock) k_beta(const Args a) {
-  const int m = a.m0 + blockIdx.x;
-  const Member mb = member_at(a, m);
-  __shared__ double red[4 * 6];
-  const int tid = threadIdx.x;
-  const int nseg = a.nseg;
-  const int per_comp = a.M * nseg;
-  // inner_product(tend(old), tend(new)) and (tend(new), tend(new)) over the
-  // (row, segment) partials of du, dv and dgd (types_mod.F90:347-371); the
-  // loads of 8 strides are issued together.
-  double q[6] = {0, 0, 0, 0, 0, 0};
-  for (int k0 = 0; k0 < per_comp; k0 += 8 * kBlock) {
-    double x[8][6];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int k = k0 + u * kBlock + tid;
-      const int row = k / nseg, sg = k - (k / nseg) * nseg;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        x[u][2 * c] = x[u][2 * c + 1] = 0.0;
-        if (k < per_comp) {
-          const double *pp = mb.part() + ((size_t)(row * 3 + c) * nseg + sg) * 2;
-          x[u][2 * c] = pp[0];
-          x[u][2 * c + 1] = pp[1];
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) q[c] += x[u][c];
-  }
-  block_sum_n<6>(q, red);
-  // du, then dv, then dgd
-  const double ip1 = (q[0] + q[2]) + q[4];
-  const double ip2 = (q[1] + q[3]) + q[5];
-  if (tid == 0) mb.beta()[0] = (a.qcon && ip1 != 0.0 && ip2 != 0.0) ? ip1 / ip2 : 1.0;
-}
-
-// ---------------------------------------------------------------------------
-// corrector update (predict_correct, dycore_mod.F90:789-790), pointwise
-// ---------------------------------------------------------------------------
-
-template <typename CT>
-__global__ void __launch_bounds__(kBlock) k_correct(const Args a) {
-  const int t = xcd_tile(a.ntiles);
-  if (t < 0) return;
-  const int N = a.N, M = a.M, P = a.pitch, nseg = a.nseg;
-  const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
-  const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
-  const Member mb = member_at(a, a.m0 + mloc);
-  const int i = g * kSeg + threadIdx.x;
-  if (threadIdx.x >= kSeg || i >= N) return;
-  const double dtb = a.dt * mb.beta()[0];
-  const size_t o = (size_t)j * P + i;
-  mb.st(a.s_out, FGD)[o] = mb.st(a.s0, FGD)[o] + dtb * ((const double *)mb.tc(CDGD))[o];
-  mb.st(a.s_out, FU)[o] = mb.st(a.s0, FU)[o] + dtb * (double)((const CT *)mb.tc(CDU))[o];
-  if (j <= M - 2) mb.st(a.s_out, FV)[o] = mb.st(a.s0, FV)[o] + dtb * (double)((const CT *)mb.tc(CDV))[o];
-}
-
-// ---------------------------------------------------------------------------
 // beta + corrector update (predict_correct, dycore_mod.F90:784-790)
 // ---------------------------------------------------------------------------
 
 // upd_wpm workgroups per member.  Each forms the member's beta from the
-// phase-C partials exactly as k_beta does (same loads, same order, so the
-// same bits), then applies S_out = S0 + dt beta Tc to its band of rows.
+// phase-C partials (same loads, same order in every workgroup, so the same
+// bits), then applies S_out = S0 + dt beta Tc to its band of rows.
 template <typename CT>
 __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   const int wpm = a.upd_wpm;
@@ -443,7 +379,8 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   };
   // the first batch of the band does not depend on beta: issue it first
   load(0);
-  // beta from the phase-C partials, the same loads and order as k_beta
+  // beta from the phase-C partials: inner_product(tend(old), tend(new)) and
+  // (tend(new), tend(new)) (types_mod.F90:347-371) summed du, then dv, then dgd
   const int per_comp = M * nseg;
   double q[6] = {0, 0, 0, 0, 0, 0};
   for (int k0 = 0; k0 < per_comp; k0 += 8 * kBlock) {
@@ -929,10 +866,6 @@ hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStrea
   return prec == 0 ? launch_tend_t<double>(a, pass, in_full, s) : launch_tend_t<float>(a, pass, in_full, s);
 }
 
-hipError_t launch_beta(const Args &a, int nmem, hipStream_t s) {
-  hipLaunchKernelGGL(k_beta, dim3(nmem), dim3(kBlock), 0, s, a);
-  return hipGetLastError();
-}
 
 hipError_t launch_isp(const Args &a, int prec, hipStream_t s) {
   dim3 g(row_grid(a.ntiles)), b(kBlock);
@@ -952,14 +885,6 @@ hipError_t launch_update(const Args &a, int prec, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_correct(const Args &a, int prec, hipStream_t s) {
-  dim3 g(row_grid(a.ntiles)), b(kBlock);
-  if (prec == 0)
-    hipLaunchKernelGGL(k_correct<double>, g, b, 0, s, a);
-  else
-    hipLaunchKernelGGL(k_correct<float>, g, b, 0, s, a);
-  return hipGetLastError();
-}
 
 hipError_t launch_diffuse(const Args &a, int prec, hipStream_t s) {
   dim3 g(row_grid(a.ntiles)), b(kBlock);

@@ -19,8 +19,6 @@ size_t tend_lds_bytes(const Args &a, size_t cts);
 
 // prec: 0 = FP64 plain, 1 = mixed (FP32 compute type)
 hipError_t launch_tend(const Args &a, int prec, int pass, bool in_full, hipStream_t s);
-hipError_t launch_beta(const Args &a, int nmem, hipStream_t s);
-hipError_t launch_correct(const Args &a, int prec, hipStream_t s);
 // beta of predict_correct and the corrector update in one launch
 // (upd_wpm workgroups for each of nmem_launch members)
 hipError_t launch_update(const Args &a, int prec, hipStream_t s);
