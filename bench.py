@@ -36,12 +36,11 @@ def tend_launch_bytes(N, M, prec):
     """Algorithmic HBM bytes of one launch of the tendency kernel in the
     second predictor of a fast-pass call (S2 = S0 + dt/2 L(S1), keep Tb):
     read S1 (gd, U, V), ghs and S0 (gd, U, V); write S2 (gd, U, V) and Tb
-    (du, dv in the compute type, dgd in FP64).  gd, U, u live on N x M full
-    rows, V on N x (M-1) half rows; FP64 state."""
+    (du, dv, dgd).  gd, U, u live on N x M full rows, V on N x (M-1) half
+    rows; state and stored tendencies are FP64 in both precision modes."""
     full, half = N * M, N * (M - 1)
     state = 8 * (full + full + half)            # gd, U, V
-    tsz = 8 if prec == "fp64" else 4
-    tend = tsz * (full + half) + 8 * full       # du, dv, dgd
+    tend = 8 * (full + half) + 8 * full         # du, dv, dgd
     return state + 8 * full + state + state + tend
 
 

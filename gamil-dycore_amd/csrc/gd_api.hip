@@ -35,7 +35,7 @@ struct gd_handle {
   gd_config cfg{};
   std::vector<int32_t> cutoff;
   Geometry g;
-  int nmem = 1, prec = 0, N = 0, M = 0, P = 0, cts = 8;
+  int nmem = 1, prec = 0, N = 0, M = 0, P = 0;
   int nact = 1;                  // members 0..nact-1 are stepped
   int device = 0;
   hipStream_t stream = nullptr;
@@ -452,19 +452,8 @@ hipError_t d2h_rows(gd_handle *h, double *dst, const double *src, int rows, int6
 
 int read_tend(gd_handle *h, const Member &mb, int comp, double *out, int rows, int64_t ld) {
   if (!out) return GD_OK;
-  std::vector<char> buf((size_t)rows * h->P * (comp == CDGD ? 8 : h->cts));
-  GD_HIP(h, hipMemcpyAsync(buf.data(), mb.tc(comp), buf.size(), hipMemcpyDeviceToHost, h->stream));
+  GD_HIP(h, d2h_rows(h, out, (const double *)mb.tc(comp), rows, ld));
   GD_HIP(h, hipStreamSynchronize(h->stream));
-  for (int j = 0; j < rows; ++j)
-    for (int i = 0; i < h->N; ++i) {
-      const size_t k = (size_t)j * h->P + i;
-      double v;
-      if (comp == CDGD || h->cts == 8)
-        v = ((const double *)buf.data())[k];
-      else
-        v = (double)((const float *)buf.data())[k];
-      out[(size_t)j * ld + i] = v;
-    }
   return GD_OK;
 }
 
@@ -494,7 +483,6 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->nmem = n_members;
   h->nact = n_members;
   h->prec = cfg->precision == GD_PREC_FP64 ? 0 : 1;
-  h->cts = h->prec == 0 ? 8 : 4;
   h->N = cfg->num_lon;
   h->M = cfg->num_lat;
   h->P = ((h->N + 31) / 32) * 32;
@@ -540,7 +528,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   const size_t nslots = h->slots.size();
   const size_t a_nseg = (h->N + kSeg - 1) / kSeg;
   h->fs = al(fld * 8);
-  h->cs = al(fld * h->cts);
+  h->cs = al(fld * 8);
   h->off_part = (size_t)(NSLOT * NFLD + 5) * h->fs + 4 * h->cs;
   h->off_dpart = h->off_part + al((size_t)h->M * 3 * a_nseg * 2 * 8);
   h->off_sx = h->off_dpart + al((size_t)h->M * a_nseg * 4 * 8);

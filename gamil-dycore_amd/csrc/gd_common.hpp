@@ -66,20 +66,15 @@ __device__ __forceinline__ double fld_base(const Member &mb, int slot, int comp,
   return mb.st(slot, comp == CDU ? FU : (comp == CDV ? FV : FGD))[o];
 }
 
-template <typename CT>
+// Stored tendencies Tb, Tc are FP64 in every precision mode: the FP32
+// values of mixed mode widen exactly, filtered rows keep the FP64 rescale,
+// and the wider loads measured faster than FP32 ones in the corrector.
 __device__ __forceinline__ double tend_load(const Member &mb, bool tb, int comp, size_t o) {
-  void *p = tb ? mb.tb(comp) : mb.tc(comp);
-  if (comp == CDGD) return ((const double *)p)[o];
-  return (double)((const CT *)p)[o];
+  return (tb ? mb.tb(comp) : mb.tc(comp))[o];
 }
 
-template <typename CT>
 __device__ __forceinline__ void tend_store(const Member &mb, bool tb, int comp, size_t o, double v) {
-  void *p = tb ? mb.tb(comp) : mb.tc(comp);
-  if (comp == CDGD)
-    ((double *)p)[o] = v;
-  else
-    ((CT *)p)[o] = (CT)v;
+  (tb ? mb.tb(comp) : mb.tc(comp))[o] = v;
 }
 
 // Epilogue of one tendency value of a predict/correct phase (not diffusion).
@@ -91,11 +86,11 @@ __device__ __forceinline__ void phase_epilogue(const Args &a, const Member &mb, 
   const int f = comp == CDU ? FU : (comp == CDV ? FV : FGD);
   if (a.phase == PH_A || a.phase == PH_B) {
     mb.st(a.s_out, f)[o] = mb.st(a.s0, f)[o] + a.hdt * T;
-    if (a.phase == PH_B) tend_store<CT>(mb, true, comp, o, T);
+    if (a.phase == PH_B) tend_store(mb, true, comp, o, T);
   } else {
-    tend_store<CT>(mb, false, comp, o, T);
+    tend_store(mb, false, comp, o, T);
     if (a.phase == PH_C) {
-      double tbv = tend_load<CT>(mb, true, comp, o);
+      double tbv = tend_load(mb, true, comp, o);
       p1 += tbv * T * w;
       p2 += T * T * w;
     }
@@ -108,7 +103,7 @@ __device__ __forceinline__ void phase_epilogue(const Args &a, const Member &mb, 
 template <typename CT>
 __device__ __forceinline__ double phase_pre(const Args &a, const Member &mb, int comp, size_t o) {
   if (a.phase == PH_A || a.phase == PH_B) return fld_base(mb, a.s0, comp, o);
-  if (a.phase == PH_C) return tend_load<CT>(mb, true, comp, o);
+  if (a.phase == PH_C) return tend_load(mb, true, comp, o);
   return 0.0;
 }
 
@@ -119,9 +114,9 @@ __device__ __forceinline__ void phase_store(const Args &a, const Member &mb, int
   const int f = comp == CDU ? FU : (comp == CDV ? FV : FGD);
   if (a.phase == PH_A || a.phase == PH_B) {
     mb.st(a.s_out, f)[o] = pre + a.hdt * T;
-    if (a.phase == PH_B) tend_store<CT>(mb, true, comp, o, T);
+    if (a.phase == PH_B) tend_store(mb, true, comp, o, T);
   } else {
-    tend_store<CT>(mb, false, comp, o, T);
+    tend_store(mb, false, comp, o, T);
     if (a.phase == PH_C) {
       p1 += pre * T * w;
       p2 += T * T * w;

@@ -49,7 +49,7 @@ struct RowCoef {
 // pointers arithmetically from kernel arguments (no table in HBM to load):
 //   k * fs, k = slot * NFLD + field   state fields (double)
 //   25 fs ghs, 26 fs Tb dgd, 27 fs Tc dgd, 28 fs div, 29 fs vor (double)
-//   30 fs + q cs                      Tb du, Tb dv, Tc du, Tc dv (compute type)
+//   30 fs + q cs                      Tb du, Tb dv, Tc du, Tc dv (double)
 //   off_part, off_dpart, off_sx, off_beta  beta partials [M][3][nseg][2], diag
 //                                     partials [M][nseg][4], staged filter rows
 //                                     [nslots][pitch], beta of the last call
@@ -59,11 +59,11 @@ struct Member {
   double *diag_out;  // [cap][3]: mass, energy, beta per step
   __host__ __device__ double *st(int slot, int f) const { return (double *)(b + (size_t)(slot * NFLD + f) * fs); }
   __host__ __device__ double *ghs() const { return (double *)(b + (size_t)(NSLOT * NFLD) * fs); }
-  __host__ __device__ void *tb(int c) const {
-    return c == CDGD ? (void *)(b + (size_t)(NSLOT * NFLD + 1) * fs) : (void *)(b + (size_t)(NSLOT * NFLD + 5) * fs + (size_t)c * cs);
+  __host__ __device__ double *tb(int c) const {
+    return (double *)(c == CDGD ? b + (size_t)(NSLOT * NFLD + 1) * fs : b + (size_t)(NSLOT * NFLD + 5) * fs + (size_t)c * cs);
   }
-  __host__ __device__ void *tc(int c) const {
-    return c == CDGD ? (void *)(b + (size_t)(NSLOT * NFLD + 2) * fs) : (void *)(b + (size_t)(NSLOT * NFLD + 5) * fs + (size_t)(2 + c) * cs);
+  __host__ __device__ double *tc(int c) const {
+    return (double *)(c == CDGD ? b + (size_t)(NSLOT * NFLD + 2) * fs : b + (size_t)(NSLOT * NFLD + 5) * fs + (size_t)(2 + c) * cs);
   }
   __host__ __device__ double *div() const { return (double *)(b + (size_t)(NSLOT * NFLD + 3) * fs); }
   __host__ __device__ double *vor() const { return (double *)(b + (size_t)(NSLOT * NFLD + 4) * fs); }

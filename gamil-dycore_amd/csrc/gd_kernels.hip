@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   // the FP64 accumulator (slot SP2 u, v fields and the Tb dgd field)
   const bool isp = a.upd_isp != 0;
   const double *tcg = isp ? (const double *)mb.tb(CDGD) : (const double *)mb.tc(CDGD);
-  const CT *tcu = (const CT *)mb.tc(CDU), *tcv = (const CT *)mb.tc(CDV);
+  const double *tcu = mb.tc(CDU), *tcv = mb.tc(CDV);
   const double *rcu = mb.st(SP2, Fu), *rcv = mb.st(SP2, Fv);
   double *og = mb.st(a.s_out, FGD), *ou = mb.st(a.s_out, FU), *ov = mb.st(a.s_out, FV);
   constexpr int U = 4;  // points per thread per batch
@@ -372,9 +372,9 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
       g0[k] = on[k] ? s0g[off[k]] : 0.0;
       u0[k] = on[k] ? s0u[off[k]] : 0.0;
       tg[k] = on[k] ? tcg[off[k]] : 0.0;
-      tu[k] = on[k] ? (isp ? rcu[off[k]] : (double)tcu[off[k]]) : 0.0;
+      tu[k] = on[k] ? (isp ? rcu[off[k]] : tcu[off[k]]) : 0.0;
       v0[k] = hv[k] ? s0v[off[k]] : 0.0;
-      tv[k] = hv[k] ? (isp ? rcv[off[k]] : (double)tcv[off[k]]) : 0.0;
+      tv[k] = hv[k] ? (isp ? rcv[off[k]] : tcv[off[k]]) : 0.0;
     }
   };
   // the first batch of the band does not depend on beta: issue it first
@@ -464,8 +464,8 @@ __global__ void __launch_bounds__(kBlock) k_isp(const Args a) {
   double p[6] = {0, 0, 0, 0, 0, 0};
   if (tid < W) {
     const double tg = ((const double *)mb.tc(CDGD))[o];
-    const double tu = (double)((const CT *)mb.tc(CDU))[o];
-    const double tv = hv ? (double)((const CT *)mb.tc(CDV))[o] : 0.0;
+    const double tu = mb.tc(CDU)[o];
+    const double tv = hv ? mb.tc(CDV)[o] : 0.0;
     if (mode == ISP_SLOW_SAVE) {
       sl_g[o] = tg;
       sl_u[o] = tu;

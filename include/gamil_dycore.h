@@ -46,8 +46,9 @@ enum { GD_ADV_CENTER_DIFF = 0, GD_ADV_UPWIND = 1, GD_ADV_WENO = 2 };
 enum { GD_PASS_ALL = 0, GD_PASS_FAST = 1, GD_PASS_SLOW = 2 };
 enum {
   GD_PREC_FP64 = 0,        /* all arithmetic in FP64 (reference -fdefault-real-8) */
-  GD_PREC_MIXED = 1,       /* FP32 stencils/tendencies/filter; FP64 state update,
-                              pole sums, mass flux divergence and integrals      */
+  GD_PREC_MIXED = 1,       /* FP32 momentum stencils; FP64 state update, stored
+                              tendencies, pole sums, mass flux divergence, polar
+                              filter and integrals                               */
   GD_PREC_MIXED_BF16 = 2   /* accepted for namelist compatibility; runs as MIXED   */
 };
 
