@@ -150,6 +150,17 @@ def _time_steps(d, steps, warmup):
     return time.perf_counter() - t0
 
 
+def _tend_roofline(N, M, members, us):
+    """HBM roofline of the fast-pass second-predictor tendency launch over
+    all members (its algorithmic bytes over its replayed duration)."""
+    nbytes = tend_launch_bytes(N, M, "mixed") * members
+    achieved = nbytes / (us * 1e-6) / 1e9
+    return {"bound": "hbm", "kernel": "k_tend<float, fast pass> (second predictor)",
+            "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4), "traffic": None, "bytes_per_launch": int(nbytes),
+            "avg_launch_us": round(us, 3)}
+
+
 def other_workload(gd, args):
     base = {"n_gpus": 1, "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None,
             "dtype": "f32/f64", "roofline": None, "cpu_baseline": None}
@@ -161,14 +172,16 @@ def other_workload(gd, args):
         steps = min(args.steps, 1800)
         ics = [gd.ic_balanced(N, M, seed=20243, nmax=20, pert_seed=5000 + k, pert_nmax=20, pert_rms=5.0)
                for k in range(mem)]
-        out = {}
+        out, tend_us = {}, {}
         for prec in ("mixed", "fp64"):
             d = _cfg(gd, N, M, dt, 1.0e5, prec, members=mem)
             for k, ic in enumerate(ics):
                 d.set_state(*ic, member=k)
             el = _time_steps(d, steps, args.warmup)
+            tend_us[prec] = d.time_kernel("tend", 50)
             d.close()
             out[prec] = mem * steps * dt / 86400.0 / el
+        base["roofline"] = _tend_roofline(N, M, mem, tend_us["mixed"])
         return dict(base, metric="member-days/sec, 64-member ensemble at 360x181 (mixed vs FP64)",
                     value=round(out["mixed"], 3), unit="member-days/s", steps=steps,
                     ms_per_step=round(mem * steps * dt / 86400.0 / out["mixed"] / steps * 1e3, 4),
@@ -181,13 +194,15 @@ def other_workload(gd, args):
         N, M, dt = 1440, 721, 60.0
         steps = min(args.steps, 2880)
         ic = gd.ic_balanced(N, M, seed=20242, nmax=80)
-        out = {}
+        out, tend_us = {}, {}
         for prec in ("mixed", "fp64"):
             d = _cfg(gd, N, M, dt, 6.25e3, prec)
             d.set_state(*ic)
             el = _time_steps(d, steps, args.warmup)
+            tend_us[prec] = d.time_kernel("tend", 100)
             d.close()
             out[prec] = steps * dt / 86400.0 / el
+        base["roofline"] = _tend_roofline(N, M, 1, tend_us["mixed"])
         return dict(base, metric="model-days/sec at 1440x721 (mixed vs FP64)",
                     value=round(out["mixed"], 4), unit="model-days/s", steps=steps,
                     ms_per_step=round(dt / 86400.0 / out["mixed"] * 1e3, 4), scaling="weak",
