@@ -414,6 +414,8 @@ class Dycore:
 
     def copy_member(self, dst, src):
         _check(lib().gd_copy_member(self.h, int(dst), int(src)), self.h)
+        if src in self._ghs:
+            self._ghs[dst] = self._ghs[src]
 
     # -- operator seams ----------------------------------------------------
     def space_operators(self, pass_="all", member=0):

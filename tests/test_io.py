@@ -253,3 +253,19 @@ def test_schedule_rejects_bad_settings(exe, tmp_path, extra, msg):
     r, ev, steps = schedule(exe, tmp_path, 1200, extra)
     assert r.returncode == 1
     assert "[Error]: " + msg in r.stdout
+
+
+def test_schedule_ensemble_names_and_output_dir(exe, tmp_path):
+    """Ensemble members write their own files (.mNNN); output_dir prefixes
+    every path; write_output = .false. writes nothing."""
+    r, ev, steps = schedule(exe, tmp_path, 3600, "run_hours = 12\nhistory_periods = '6 hours'\n"
+                                                 "restart_period = '12 hours'\nensemble_members = 2\n"
+                                                 "output_dir = '/data/run1'")
+    assert r.returncode == 0, r.stdout
+    hist = [e[3] for e in ev if e[0] == "history"]
+    assert hist == ["/data/run1/c.m001.h0.h0.0001-01-01T06:00:00.nc",
+                    "/data/run1/c.m002.h0.h0.0001-01-01T06:00:00.nc",
+                    "/data/run1/c.m001.h0.h0.0001-01-01T12:00:00.nc",
+                    "/data/run1/c.m002.h0.h0.0001-01-01T12:00:00.nc"]
+    r, ev, steps = schedule(exe, tmp_path, 3600, "run_hours = 12\nwrite_output = .false.")
+    assert r.returncode == 0 and ev == [] and steps == 12
