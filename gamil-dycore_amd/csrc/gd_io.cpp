@@ -382,7 +382,7 @@ int gd_restart_read(const char *path, int32_t N, int32_t M, double *u, double *v
       RVar var;
       var.name = r.name();
       const uint32_t nd = r.u32();
-      for (uint32_t q = 0; q < nd; ++q) var.dimids.push_back(r.u32());
+      for (uint32_t q = 0; q < nd && r.ok; ++q) var.dimids.push_back(r.u32());
       if (!read_atts(r, nullptr)) return io_fail(std::string(path) + ": bad variable attributes");
       var.type = r.u32();
       r.u32();  // vsize
@@ -410,7 +410,7 @@ int gd_restart_read(const char *path, int32_t N, int32_t M, double *u, double *v
       if (count(v) != (uint64_t)rows * N) return io_fail(std::string(name) + " has the wrong size");
       // the last record
       const uint64_t at = v.begin + (is_rec(v) ? (uint64_t)(numrecs - 1) * recsize : 0);
-      if (at + count(v) * 8 > r.b.size()) return io_fail(std::string(path) + ": truncated data");
+      if (at > r.b.size() || count(v) * 8 > r.b.size() - at) return io_fail(std::string(path) + ": truncated data");
       for (int j = 0; j < rows; ++j)
         for (int i = 0; i < N; ++i) dst[(size_t)j * ld + i] = r.dbl(at + ((size_t)j * N + i) * 8);
       return GD_OK;

@@ -269,3 +269,24 @@ def test_schedule_ensemble_names_and_output_dir(exe, tmp_path):
                     "/data/run1/c.m002.h0.h0.0001-01-01T12:00:00.nc"]
     r, ev, steps = schedule(exe, tmp_path, 3600, "run_hours = 12\nwrite_output = .false.")
     assert r.returncode == 0 and ev == [] and steps == 12
+
+
+def test_restart_reader_rejects_corrupt_headers(gd, tmp_path):
+    """Counts and offsets read from the file are bounds-checked: a corrupted
+    header fails with an error instead of reading past the file."""
+    p = tmp_path / "ok.nc"
+    u, v, g, ghs, _, _ = fields(8, 5)
+    gd.write_restart(str(p), u, v, g, ghs)
+    raw = bytearray(open(p, "rb").read())
+    rng = np.random.default_rng(0)
+    for trial in range(200):
+        b = bytearray(raw)
+        for _ in range(4):
+            k = int(rng.integers(4, 400))
+            b[k] = int(rng.integers(0, 256))
+        q = tmp_path / f"bad{trial}.nc"
+        q.write_bytes(bytes(b))
+        try:
+            gd.read_restart(str(q), 8, 5)
+        except gd.DycoreError:
+            pass
