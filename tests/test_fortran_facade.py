@@ -40,12 +40,12 @@ def exe():
 def test_committed_namelists_parse(exe, gd, path):
     out = check(path)
     nlon, nlat = [int(x) for x in out["grid"].split("x")]
-    text = open(path).read()
+    text = "\n".join(ln.split("!")[0] for ln in open(path).read().splitlines())
     m = re.search(r"zonal_tend_filter_lat0\s*=\s*([0-9.]+)", text)
     if m:
         expect = gd.cutoff_from_lat0(nlon, nlat, float(m.group(1)))
         assert [int(c) for c in out["filter cutoffs"].split()] == expect
-    days = int(re.search(r"run_days\s*=\s*(\d+)", text).group(1))
+    days = int(re.search(r"\b(?:run_)?days\s*=\s*(\d+)", text).group(1))
     dt = float(re.search(r"time_step_size\s*=\s*([0-9.]+)", text).group(1))
     assert int(out["steps"]) == int(np.ceil(days * 86400 / dt))
 
