@@ -128,6 +128,9 @@ def test_rejects_bad_state_and_config(gd):
         gd.Dycore(N, M, 600.0, zonal_tend_filter_cutoff_wavenumber=[0, 4])
     with pytest.raises(gd.DycoreError):
         gd.Dycore(N, M, 600.0, zonal_tend_filter_cutoff_wavenumber=[N // 2])
+    with pytest.raises(gd.DycoreError) as ei:  # filter rows longer than 4096 points
+        gd.Dycore(4104, 9, 60.0, zonal_tend_filter_cutoff_wavenumber=[4])
+    assert ei.value.code == gd.GD_ERR_NOT_SUPPORTED
 
 
 def test_time_kernel_leaves_state_untouched(gd):

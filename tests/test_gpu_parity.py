@@ -66,18 +66,19 @@ def test_space_operators_unfiltered_rows_bitwise(gd, oracle_mod):
     d.close()
 
 
-# Grid sizes exercise every FFT code path of the device filter: radix 4 x 9
-# (72 -> 36), radix 16 x 4 (128 -> 64), radix 9 x 5 with an odd half length
+# Grid sizes exercise every FFT code path of the device filter: radix 4/3
+# (72 -> 36), radix 8/8 (128 -> 64), radix 3/5 with an odd half length
 # (90 -> 45), odd N (75, full complex transform), radix 7 (70 -> 35), a
 # prime half length handled by the direct O(R^2) pass (94 -> 47), radix
-# 16 x 9 (288 -> 144), the 720-longitude grid (8 x 9 x 5) and the
-# 1440-longitude grid (16 x 9 x 5).
+# 8 x 2 x 3 x 3 (288 -> 144), the 720- and 1440-longitude grids, and the
+# 16-points-per-thread path up to the largest supported row (2560, 4096).
 @pytest.mark.parametrize("N,M,cut", [(72, 37, [6, 5, 4, 3, 2, 1]), (128, 65, [30, 20, 10, 60, 61]),
                                      (90, 46, [44, 1, 43]), (75, 38, [36, 20, 1, 7]),
                                      (70, 36, [34, 2, 17]), (94, 48, [46, 3, 12]),
                                      (288, 145, [143, 100, 71, 9, 1]),
                                      (720, 361, [359, 300, 200, 120, 60, 30, 15, 8, 4, 2, 1]),
-                                     (1440, 721, [719, 500, 360, 101, 33, 2])])
+                                     (1440, 721, [719, 500, 360, 101, 33, 2]),
+                                     (2560, 41, [1279, 700, 31, 2]), (4096, 33, [2047, 1000, 64, 1])])
 def test_filter_rows(gd, oracle_mod, N, M, cut):
     o, d = _pair(gd, oracle_mod, N, M, 600.0, cutoff=cut)
     rng = np.random.default_rng(1)
