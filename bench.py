@@ -275,9 +275,12 @@ def serving_workload(gd, args, base):
 
 
 def cpu_baseline(ic, nsteps):
-    """The FP64 C oracle (single thread, the reference is serial) on a bounded
-    sample of the same workload: the first nsteps steps."""
-    from oracle.oracle import Oracle, set_threads
+    """The FP64 C oracle built with the reference's optimisation level
+    (-Ofast), single thread (the reference is serial), on a bounded sample of
+    the same workload: the first nsteps steps."""
+    from oracle import oracle as orc
+    orc.use_timing_build()
+    Oracle, set_threads = orc.Oracle, orc.set_threads
     set_threads(1)
     cut = None
     import importlib
@@ -361,7 +364,8 @@ def main():
         cv, cel = cpu_baseline(ic, args.cpu_steps)
         cpu = {"value": round(cv, 6), "unit": "model-days/s", "cores": 1, "kind": "port",
                "sample": f"first {args.cpu_steps} steps of the same 720x361 FP64 workload, "
-                         f"oracle/gd_oracle.c single thread ({cel:.1f} s)"}
+                         f"oracle/gd_oracle.c built -O3 -ffast-math (the reference's -Ofast), "
+                         f"single thread ({cel:.1f} s)"}
     lat = np.array(mixed["lat"]) if mixed["lat"] else np.array([np.nan])
     lat64 = np.array(res["fp64"]["lat"]) if res["fp64"]["lat"] else np.array([np.nan])
     out = {

@@ -16,6 +16,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "_build", "libgd_oracle.so")
+_LIB_FAST = os.path.join(_HERE, "_build", "libgd_oracle_fast.so")
 _REF_FFT = os.path.join(_HERE, "_ref", "libfftpack_ref.so")
 
 ALL_PASS, FAST_PASS, SLOW_PASS = 0, 1, 2
@@ -86,6 +87,15 @@ def lib():
         L.og_run.argtypes = [vp, C.c_int, dp, dp, dp]
         _lib = L
     return _lib
+
+
+def use_timing_build():
+    """Use the -Ofast build (the reference's optimisation level) instead of
+    the strict parity build: for CPU timing only.  Call before first use."""
+    global _LIB
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIB = _LIB_FAST
 
 
 def set_threads(n):
