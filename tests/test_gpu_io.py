@@ -116,3 +116,37 @@ def test_driver_output_and_restart(gd, tmp_path):
     for k in ("u", "v", "gd", "ghs"):
         np.testing.assert_array_equal(a[k], b[k])
     assert v2["tm"][0] == var["tm"][0] and v2["te"][0] == var["te"][0]
+
+
+def test_driver_ensemble_history_members(gd, tmp_path):
+    """ensemble_members = 2 through the driver: one history file per member
+    and output time, each member's history equal to the same run alone."""
+    if not os.path.exists(EXE):
+        pytest.skip("dycore_test.exe not built")
+    nml = NML.format(case="e", extra="").replace("&balanced_sh_params\nseed = 42, nmax = 8, n_mountains = 1, "
+                                                  "mountain_seed = 3\n/\n", "")
+    nml = nml.replace("log_every = 12\n", "log_every = 12\nensemble_members = 2\nrun_hours = 12\n") \
+             .replace("run_days = 1\n", "")
+    nml += "&balanced_sh_params\nseed = 42, nmax = 8, n_mountains = 1, mountain_seed = 3, " \
+           "pert_seed = 11, pert_nmax = 6, pert_rms = 5.0\n/\n"
+    (tmp_path / "e.nml").write_text(nml)
+    r = subprocess.run([EXE, str(tmp_path / "e.nml")], capture_output=True, text=True, timeout=120,
+                       cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    files = sorted(f for f in os.listdir(tmp_path) if f.endswith(".nc"))
+    assert files == ["e.m001.h0.h0.0001-01-01T06:00:00.nc", "e.m001.h0.h0.0001-01-01T12:00:00.nc",
+                     "e.m001.r.0001-01-01T12:00:00.nc", "e.m002.h0.h0.0001-01-01T06:00:00.nc",
+                     "e.m002.h0.h0.0001-01-01T12:00:00.nc", "e.m002.r.0001-01-01T12:00:00.nc"]
+    N, M = 72, 37
+    cut = gd.cutoff_from_lat0(N, M, 75.0)
+    for m in (1, 2):
+        ic = gd.ic_balanced(N, M, seed=42, nmax=8, n_mountains=1, mountain_seed=3, pert_seed=11 + m - 1,
+                            pert_nmax=6, pert_rms=5.0)
+        with gd.Dycore(N, M, 1200.0, split_scheme="csp2", subcycles=4, use_diffusion=True,
+                       diffusion_coef=4.0e5, zonal_tend_filter_cutoff_wavenumber=cut) as d:
+            d.set_state(*ic)
+            d.step(36)
+            u, v, g = d.get_state()
+        _, var, _ = read(tmp_path / f"e.m00{m}.h0.h0.0001-01-01T12:00:00.nc")
+        np.testing.assert_array_equal(var["gh"][0], g + ic[3])
+        np.testing.assert_array_equal(var["u"][0], 0.5 * (u + np.roll(u, 1, axis=1)))
