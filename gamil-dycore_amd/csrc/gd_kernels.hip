@@ -27,13 +27,16 @@ namespace gd {
 // tendency (space_operators, dycore_mod.F90:184-598) + predictor epilogue
 // ---------------------------------------------------------------------------
 
-// LDS of one segment: G rows j-1..j+2, U rows j-1..j+1, V half rows
-// j-1..j+1 (double), u rows j, j+1 and v half rows j-1..j+1 (compute type),
-// each over the segment plus one halo point on both sides.
+// LDS of one segment: the rows whose longitude neighbours i-1, i+1 are read,
+// over the segment plus one halo point on both sides: G row j, U rows j,
+// j+1, V half rows j-1, j (double), u rows j, j+1 and v half rows j-1, j
+// (compute type).  The rows read only at a thread's own longitude (G rows
+// j-1, j+1, U row j-1, V and v half row j+1) stay in its registers, which
+// keeps the tile at 14.6 KB (mixed) / 18.7 KB (FP64) of LDS.
 constexpr int kLW = kBlock + 4;  // LDS slot li = i - i0 + 2 holds longitude i
 template <typename CT>
 __host__ __device__ constexpr size_t tend_stencil_bytes() {
-  return (size_t)(10 * sizeof(double) + 5 * sizeof(CT)) * kLW;
+  return (size_t)(5 * sizeof(double) + 4 * sizeof(CT)) * kLW;
 }
 
 // Tendency values of one point and the epilogue inputs loaded with them.
@@ -42,20 +45,22 @@ struct TendPoint {
   double pre_u, pre_v, pre_g;
 };
 
-// Segment i0 .. i0+W-1 of full row j: every global load is issued before
-// the first barrier (halo rows to LDS, the point's own geopotential terms
-// and epilogue inputs to registers), then thread tid < W evaluates its
-// point.  smem holds tend_stencil_bytes<CT>() bytes, red 24 doubles.
+// Segment i0 .. i0+W-1 of full row j: thread tid < W loads the column of
+// longitude i0+tid and evaluates that point; threads W and W+1 load the halo
+// columns i0-1 and i0+W.  Every global load is issued before the first
+// barrier (neighbour rows to LDS, the point's own rows, geopotential terms
+// and epilogue inputs to registers).  smem holds tend_stencil_bytes<CT>()
+// bytes, red 24 doubles.
 template <typename CT, int PASS, bool IN_FULL>
 __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &mb, const RowCoef &cj, int j,
                                                   int i0, int W, bool stg_u, bool stg_v, bool stg_g,
                                                   unsigned char *smem, double *red) {
   const int N = a.N, M = a.M, P = a.pitch;
-  double(*sG)[kLW] = (double(*)[kLW])smem;
-  double(*sU)[kLW] = (double(*)[kLW])(smem + 4 * kLW * sizeof(double));
-  double(*sV)[kLW] = (double(*)[kLW])(smem + 7 * kLW * sizeof(double));
-  CT(*su)[kLW] = (CT(*)[kLW])(smem + 10 * kLW * sizeof(double));
-  CT(*sv)[kLW] = (CT(*)[kLW])(smem + 10 * kLW * sizeof(double) + 2 * kLW * sizeof(CT));
+  double *sG1 = (double *)smem;                          // G row j
+  double(*sU)[kLW] = (double(*)[kLW])(smem + 1 * kLW * sizeof(double));  // U rows j, j+1
+  double(*sV)[kLW] = (double(*)[kLW])(smem + 3 * kLW * sizeof(double));  // V half rows j-1, j
+  CT(*su)[kLW] = (CT(*)[kLW])(smem + 5 * kLW * sizeof(double));          // u rows j, j+1
+  CT(*sv)[kLW] = (CT(*)[kLW])(smem + 5 * kLW * sizeof(double) + 2 * kLW * sizeof(CT));  // v j-1, j
   const int tid = threadIdx.x;
   const double *gd = mb.st(a.s_in, FGD);
   const double *Ug = mb.st(a.s_in, FU);
@@ -77,12 +82,14 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
     for (int q = tid; q < N; q += kBlock)
       pacc += (sqrt(gd[(size_t)j * P + q]) + sqrt(gd[(size_t)jn * P + q])) * Vg[(size_t)hr * P + q];
   }
-  // ---- loads: halo rows (thread e holds point i0-1+e, e < W+2 <= kBlock) ----
+  // ---- loads: the column of this thread's longitude (own point for
+  // tid < W, halo i0-1 for tid == W, halo i0+W for tid == W+1) ----
   double lg[4], lgn[2], lU[3], lV[3], lfu[2], lfv[3];
+  const int lcol = tid < W ? i0 + tid : (tid == W ? i0 - 1 : i0 + W);
+  const int lslot = tid < W ? tid + 2 : (tid == W ? 1 : W + 2);
   {
-    const int e = tid;
-    const bool on = e < W + 2;
-    const int p = wrap(i0 - 1 + e), pn = wrap(i0 + e);
+    const bool on = tid < W + 2;
+    const int p = wrap(lcol), pn = wrap(lcol + 1);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = j - 1 + r;
@@ -119,20 +126,18 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
     if (!stg_v && has_v) out.pre_v = phase_pre<CT>(a, mb, CDV, o + i);
     if (!stg_g) out.pre_g = phase_pre<CT>(a, mb, CDGD, o + i);
   }
-  // ---- LDS rows ----
+  // ---- LDS rows; the own-longitude rows stay in registers ----
+  double G[4];
+  CT u_own[2] = {(CT)0, (CT)0}, v_own3[3] = {(CT)0, (CT)0, (CT)0};  // u rows j, j+1; v half rows j-1..j+1
   if (tid < W + 2) {
-    const int li = tid + 1;
-    double G[4];
+    const int li = lslot;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      G[r] = sqrt(lg[r]);
-      sG[r][li] = G[r];
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      sU[r][li] = lU[r];
-      sV[r][li] = lV[r];
-    }
+    for (int r = 0; r < 4; ++r) G[r] = sqrt(lg[r]);
+    sG1[li] = G[1];
+    sU[0][li] = lU[1];
+    sU[1][li] = lU[2];
+    sV[0][li] = lV[0];
+    sV[1][li] = lV[1];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int row = j + r;
@@ -144,6 +149,7 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
         val = ((CT)lU[r + 1] * (CT)2) / ((CT)G[r + 1] + (CT)sqrt(lgn[r]));
       }
       su[r][li] = val;
+      u_own[r] = val;
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -155,7 +161,8 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
         else  // v = V * 2 / (G(j) + G(j+1))  (dycore_mod.F90:641-645)
           val = ((CT)lV[r] * (CT)2) / ((CT)G[r] + (CT)G[r + 1]);
       }
-      sv[r][li] = val;
+      if (r < 2) sv[r][li] = val;
+      v_own3[r] = val;
     }
   }
   double pole = 0.0;
@@ -178,10 +185,10 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
   const int li = tid + 2;
   // ---- u equation (full row j) ----
   if (has_u) {
-    const CT u0 = su[0][li], um = su[0][li - 1], up = su[0][li + 1];
+    const CT u0 = u_own[0], um = su[0][li - 1], up = su[0][li + 1];
     CT ual = 0, uat = 0, fv = 0, upgf = 0;
     if (PASS != 1) {
-      const CT U0 = (CT)sU[1][li], Um = (CT)sU[1][li - 1], Up = (CT)sU[1][li + 1];
+      const CT U0 = (CT)lU[1], Um = (CT)sU[0][li - 1], Up = (CT)sU[0][li + 1];
       const CT u1 = u0 + um, u2 = u0 + up;
       const CT kl = (CT)cj.k_uadv_lon;
       if (!upwind)
@@ -189,9 +196,9 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
       else
         ual = kl * (u2 * (U0 + Up) - bl * fabs(u2) * (Up - U0) - u1 * (U0 + Um) +
                     bl * fabs(u1) * (U0 - Um) - (u2 - u1) * U0);
-      const CT v1 = (sv[0][li] + sv[0][li + 1]) * (CT)hc_m;
-      const CT v2 = (sv[1][li] + sv[1][li + 1]) * (CT)hc_0;
-      const CT Ujp = (CT)sU[2][li], Ujm = (CT)sU[0][li];
+      const CT v1 = (v_own3[0] + sv[0][li + 1]) * (CT)hc_m;
+      const CT v2 = (v_own3[1] + sv[1][li + 1]) * (CT)hc_0;
+      const CT Ujp = (CT)lU[2], Ujm = (CT)lU[0];
       const CT kt = (CT)cj.k_uadv_lat;
       if (!upwind)
         uat = kt * (v2 * Ujp - v1 * Ujm);
@@ -200,11 +207,11 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
                     bt * fabs(v1) * (U0 - Ujm) - (v2 - v1) * U0);
     }
     if (PASS != 2) {
-      const CT Vm0 = (CT)sV[0][li], Vm1 = (CT)sV[0][li + 1];
-      const CT V00 = (CT)sV[1][li], V01 = (CT)sV[1][li + 1];
+      const CT Vm0 = (CT)lV[0], Vm1 = (CT)sV[0][li + 1];
+      const CT V00 = (CT)lV[1], V01 = (CT)sV[1][li + 1];
       fv = (CT)0.25 * ((CT)cj.f + (CT)cj.c * u0) * ((CT)cj.c1 * (Vm0 + Vm1) + (CT)cj.c2 * (V00 + V01));
       const double dphi = g_ip + h_ip - g_i - h_i;
-      upgf = (CT)0.5 * ((CT)sG[1][li] + (CT)sG[1][li + 1]) / (CT)cj.fdlon * (CT)dphi;
+      upgf = (CT)0.5 * ((CT)G[1] + (CT)sG1[li + 1]) / (CT)cj.fdlon * (CT)dphi;
     }
     CT d;
     if (PASS == 0)
@@ -220,19 +227,19 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
     CT val = 0, vat = 0, fu = 0, vpgf = 0;
     if (PASS != 1) {
       const CT u1 = su[0][li - 1] + su[1][li - 1];
-      const CT u2 = su[0][li] + su[1][li];
-      const CT V0 = (CT)sV[1][li], Vm = (CT)sV[1][li - 1], Vp = (CT)sV[1][li + 1];
+      const CT u2 = u_own[0] + u_own[1];
+      const CT V0 = (CT)lV[1], Vm = (CT)sV[1][li - 1], Vp = (CT)sV[1][li + 1];
       const CT kl = (CT)cj.k_vadv_lon;
       if (!upwind)
         val = kl * (u2 * Vp - u1 * Vm);
       else
         val = kl * (u2 * (V0 + Vp) - bl * fabs(u2) * (Vp - V0) - u1 * (V0 + Vm) +
                     bl * fabs(u1) * (V0 - Vm) - (u2 - u1) * V0);
-      const CT vj = sv[1][li];
-      const CT v1 = vj * (CT)hc_0 + sv[0][li] * (CT)hc_m;
-      const CT v2 = vj * (CT)hc_0 + sv[2][li] * (CT)hc_p;
-      const CT Vjp = (CT)sV[2][li];  // zero beyond the last half row
-      const CT Vjm = (CT)sV[0][li];  // zero before the first half row
+      const CT vj = v_own3[1];
+      const CT v1 = vj * (CT)hc_0 + v_own3[0] * (CT)hc_m;
+      const CT v2 = vj * (CT)hc_0 + v_own3[2] * (CT)hc_p;
+      const CT Vjp = (CT)lV[2];  // zero beyond the last half row
+      const CT Vjm = (CT)lV[0];  // zero before the first half row
       const CT kt = (CT)cj.k_vadv_lat;
       if (!upwind)
         vat = kt * (v2 * Vjp - v1 * Vjm);
@@ -242,13 +249,13 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
     }
     if (PASS != 2) {
       const CT f0 = (CT)cj.f, c0 = (CT)cj.c, fp = (CT)f_p, cp = (CT)c_p;
-      const CT a1 = (f0 + c0 * su[0][li]) * (CT)sU[1][li];
-      const CT a2 = (f0 + c0 * su[0][li - 1]) * (CT)sU[1][li - 1];
-      const CT a3 = (fp + cp * su[1][li]) * (CT)sU[2][li];
-      const CT a4 = (fp + cp * su[1][li - 1]) * (CT)sU[2][li - 1];
+      const CT a1 = (f0 + c0 * u_own[0]) * (CT)lU[1];
+      const CT a2 = (f0 + c0 * su[0][li - 1]) * (CT)sU[0][li - 1];
+      const CT a3 = (fp + cp * u_own[1]) * (CT)lU[2];
+      const CT a4 = (fp + cp * su[1][li - 1]) * (CT)sU[1][li - 1];
       fu = (CT)0.25 * (a1 + a2 + a3 + a4);
       const double dphi = g_n + h_n - g_i - h_i;
-      vpgf = (CT)0.5 * ((CT)sG[1][li] + (CT)sG[2][li]) / (CT)cj.hdlat * (CT)cj.hcos * (CT)dphi;
+      vpgf = (CT)0.5 * ((CT)G[1] + (CT)G[2]) / (CT)cj.hdlat * (CT)cj.hcos * (CT)dphi;
     }
     CT d;
     if (PASS == 0)
@@ -265,9 +272,9 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
     if (j == 0 || j == M - 1) {
       mdp = pole;
     } else {
-      const double Gi = sG[1][li], Gip = sG[1][li + 1], Gim = sG[1][li - 1];
-      mdl = ((Gi + Gip) * sU[1][li] - (Gi + Gim) * sU[1][li - 1]) * 0.5 / cj.fdlon;
-      mdp = ((Gi + sG[2][li]) * sV[1][li] * hc_0 - (Gi + sG[0][li]) * sV[0][li] * hc_m) * 0.5 / cj.fdlat;
+      const double Gi = G[1], Gip = sG1[li + 1], Gim = sG1[li - 1];
+      mdl = ((Gi + Gip) * lU[1] - (Gi + Gim) * sU[0][li - 1]) * 0.5 / cj.fdlon;
+      mdp = ((Gi + G[2]) * lV[1] * hc_0 - (Gi + G[0]) * lV[0] * hc_m) * 0.5 / cj.fdlat;
     }
     out.dgd = -mdl - mdp;
   }
