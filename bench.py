@@ -306,7 +306,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=7200)
     ap.add_argument("--warmup", type=int, default=360)
-    ap.add_argument("--repeats", type=int, default=3, help="full 10-day runs for p50/p99 latency")
+    ap.add_argument("--repeats", type=int, default=5, help="full 10-day runs for p50/p99 latency")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-members", type=int, default=8,
