@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-value",
 ]
-HIP_SOURCES = ["gd_kernels.hip", "gd_filter.hip", "gd_api.hip", "gd_host.cpp", "gd_io.cpp",
+HIP_SOURCES = ["gd_kernels.hip", "gd_filter.hip", "gd_phase.hip", "gd_api.hip", "gd_host.cpp", "gd_io.cpp",
                "gd_serve.cpp"]
 
 

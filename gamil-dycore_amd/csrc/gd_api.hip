@@ -48,9 +48,14 @@ struct gd_handle {
   StageSlot *dslots = nullptr;
   std::vector<FiltRow> frows;    // filter workgroups: (member, latitude) with staged rows
   FiltRow *dfrows = nullptr;
-  double *dtwid = nullptr;       // [N][2] cos, sin of 2 pi k/N
+  double *dtwid = nullptr;       // [N/2+1][2] cos, sin of 2 pi k/N
   int fft_L = 1, fft_half = 1;
   std::vector<int> fft_rad;
+  // fused phase launches (k_phase): the filter rows inside the tendency
+  // launch; the row tiles cover the row ranges far_lo/far_cnt
+  bool fused = false;
+  int nfar_rng = 0, nfar_rows = 0;
+  int far_lo[kMaxFarRng] = {}, far_cnt[kMaxFarRng] = {};
   int32_t *dstep = nullptr;
   double *ddiag = nullptr;  // [nmem][cap][3]
   int diag_cap = 512;
@@ -155,6 +160,27 @@ hipError_t enqueue_filter(gd_handle *h, Args a) {
   return run(h, "filter", [&] { return launch_filter(a, h->prec, h->stream); });
 }
 
+// One tendency phase (a.phase) of the active members with its filter rows:
+// one k_phase launch, or k_tend then k_filter.
+hipError_t enqueue_phase(gd_handle *h, Args a, int pass, bool in_full) {
+  a.ntiles = h->nact * h->M * a.nseg;
+  if (!h->fused) {
+    hipError_t e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, in_full, h->stream); });
+    if (e != hipSuccess) return e;
+    return enqueue_filter(h, a);
+  }
+  a.nnear = (int)(h->frows.size() / h->nmem) * h->nact;
+  a.nnear_pad = ((a.nnear + kXcd - 1) / kXcd) * kXcd;
+  a.nfar_rng = h->nfar_rng;
+  a.nfar_rows = h->nfar_rows;
+  for (int k = 0; k < kMaxFarRng; ++k) {
+    a.far_lo[k] = h->far_lo[k];
+    a.far_cnt[k] = h->far_cnt[k];
+  }
+  a.ntiles = h->nact * h->nfar_rows * a.nseg;
+  return run(h, "phase", [&] { return launch_phase(a, h->prec, pass, in_full, h->stream); });
+}
+
 // predict_correct(dt, s0 -> s_out, pass) (dycore_mod.F90:754-792)
 hipError_t enqueue_call(gd_handle *h, double dt, int pass, int s0, int s_out, int qcon) {
   Args a = base_args(h);
@@ -163,22 +189,15 @@ hipError_t enqueue_call(gd_handle *h, double dt, int pass, int s0, int s_out, in
   a.dt = dt;
   a.hdt = dt * 0.5;
   hipError_t e;
-  a.ntiles = h->nact * h->M * a.nseg;  // (member, row, segment) tiles
   // first predictor: T = L(S0), S1 = S0 + dt/2 T
   a.phase = PH_A; a.s_in = s0; a.s0 = s0; a.s_out = SP1;
-  e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, s0 == SX, h->stream); });
-  if (e != hipSuccess) return e;
-  if ((e = enqueue_filter(h, a)) != hipSuccess) return e;
+  if ((e = enqueue_phase(h, a, pass, s0 == SX)) != hipSuccess) return e;
   // second predictor: T = L(S1), S2 = S0 + dt/2 T, keep T
   a.phase = PH_B; a.s_in = SP1; a.s_out = SP2;
-  e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, false, h->stream); });
-  if (e != hipSuccess) return e;
-  if ((e = enqueue_filter(h, a)) != hipSuccess) return e;
+  if ((e = enqueue_phase(h, a, pass, false)) != hipSuccess) return e;
   // corrector tendency T = L(S2) and the beta inner-product partials
   a.phase = PH_C; a.s_in = SP2;
-  e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, false, h->stream); });
-  if (e != hipSuccess) return e;
-  if ((e = enqueue_filter(h, a)) != hipSuccess) return e;
+  if ((e = enqueue_phase(h, a, pass, false)) != hipSuccess) return e;
   a.s_out = s_out;
   return run(h, "update", [&] { return launch_update(a, h->prec, h->stream); });
 }
@@ -225,9 +244,7 @@ hipError_t enqueue_tendency(gd_handle *h, int pass, int s_in) {
   a.phase = PH_T;
   a.s_in = s_in;
   a.s0 = s_in;
-  hipError_t e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, s_in == SX, h->stream); });
-  if (e != hipSuccess) return e;
-  return enqueue_filter(h, a);
+  return enqueue_phase(h, a, pass, s_in == SX);
 }
 
 // one pointwise isp_splitting operation (k_isp mode)
@@ -515,6 +532,36 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
     delete h;
     return fail(nullptr, GD_ERR_NOT_SUPPORTED, "num_lon too large for the polar filter (at most 4096)");
   }
+  // fused phase launches: every row with a component outside the filter
+  // (and the pole rows, whose du = 0 goes through the row tiles' epilogue)
+  // in at most kMaxFarRng contiguous ranges.  GD_FUSED=0 keeps k_tend +
+  // k_filter.
+  {
+    std::vector<int> far;
+    for (int j = 0; j < h->M; ++j) {
+      const bool pole = j == 0 || j == h->M - 1;
+      const bool un = rc[j].stage[CDGD] < 0 || (j >= 1 && j <= h->M - 2 && rc[j].stage[CDU] < 0) ||
+                      (j <= h->M - 2 && rc[j].stage[CDV] < 0);
+      if (pole || un) far.push_back(j);
+    }
+    int nr = 0;
+    bool fits = true;
+    for (size_t k = 0; k < far.size(); ++k) {
+      if (k == 0 || far[k] != far[k - 1] + 1) {
+        if (nr == kMaxFarRng) { fits = false; break; }
+        h->far_lo[nr] = far[k];
+        h->far_cnt[nr] = 0;
+        ++nr;
+      }
+      ++h->far_cnt[nr - 1];
+    }
+    h->nfar_rng = nr;
+    h->nfar_rows = (int)far.size();
+    const char *env = std::getenv("GD_FUSED");
+    const bool off = env && env[0] == '0';
+    h->fused = !off && fits && !h->frows.empty() && h->N <= 4 * kBlock &&
+               phase_lds_bytes(h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
+  }
   configure_kernels((int)prop.sharedMemPerBlock);
   hipGetLastError();
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -541,17 +588,18 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->dcoef = (RowCoef *)dalloc(h, sizeof(RowCoef) * h->M, e);
   h->dslots = (StageSlot *)dalloc(h, sizeof(StageSlot) * std::max<size_t>(nslots, 1), e);
   h->dfrows = (FiltRow *)dalloc(h, sizeof(FiltRow) * std::max<size_t>(h->frows.size(), 1), e);
-  h->dtwid = (double *)dalloc(h, (size_t)h->N * 2 * 8, e);
+  h->dtwid = (double *)dalloc(h, (size_t)tw_entries(h->N) * 2 * 8, e);
   h->dstep = (int32_t *)dalloc(h, sizeof(int32_t) * h->nmem, e);
   if (!h->ddiag || !h->dcoef || !h->dslots || !h->dfrows || !h->dtwid ||
       !h->dstep) {
     gd_destroy(h);
     return fail(nullptr, GD_ERR_HIP, "device allocation failed");
   }
-  // twiddle table of the filter FFT: cos and sin of 2 pi k/N, evaluated in
-  // extended precision and rounded once to FP64
-  std::vector<double> tw((size_t)h->N * 2);
-  for (int k = 0; k < h->N; ++k) {
+  // twiddle table of the filter FFT: cos and sin of 2 pi k/N for k =
+  // 0..N/2, evaluated in extended precision and rounded once to FP64 (the
+  // kernels take W_N^k for k > N/2 as the conjugate of W_N^(N-k))
+  std::vector<double> tw((size_t)tw_entries(h->N) * 2);
+  for (int k = 0; k < tw_entries(h->N); ++k) {
     const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)h->N;
     tw[2 * k] = (double)cosl(ang);
     tw[2 * k + 1] = (double)sinl(ang);
@@ -699,9 +747,7 @@ int gd_space_operators(gd_handle *h, int32_t member, int32_t pass, double *du, d
   a.phase = PH_T;
   a.s_in = SX;
   a.s0 = SX;
-  a.ntiles = h->nmem * h->M * a.nseg;
-  GD_HIP(h, launch_tend(a, h->prec, pass, true, h->stream));
-  GD_HIP(h, enqueue_filter(h, a));
+  GD_HIP(h, enqueue_phase(h, a, pass, true));
   const Member mb = hmember(h, member);
   if ((rc = read_tend(h, mb, CDU, du, h->M, ld))) return rc;
   if ((rc = read_tend(h, mb, CDV, dv, h->M - 1, ld))) return rc;

@@ -10,8 +10,8 @@
 //        derived on the fly exactly as update_state computes them
 //        (dycore_mod.F90:636-645)
 //   P1,P2 - the two predictor states of predict_correct
-// Tendencies Tb (second predictor) and Tc (corrector) feed the beta inner
-// products; du/dv are stored in the compute type, dgd always in FP64.
+// Tendencies Tb (second predictor) and Tc (corrector), FP64 in every
+// precision mode, feed the beta inner products.
 #pragma once
 
 #include <cstdint>
@@ -29,6 +29,10 @@ enum Comp { CDU = 0, CDV = 1, CDGD = 2 };
 enum Phase { PH_A = 0, PH_B = 1, PH_C = 2, PH_T = 3, PH_D = 4, PH_P = 5 };
 
 constexpr int kMaxRad = 24;      // radix passes of the filter FFT plan
+constexpr int kMaxFarRng = 4;    // row ranges of a fused phase launch's row tiles
+
+// Entries k = 0..N/2 of the filter's twiddle table (the others are conjugates).
+__host__ __device__ inline int tw_entries(int N) { return N / 2 + 1; }
 
 // Per full row j (0-based).  Half-row entries at index j describe half row j
 // (0 for j = M-1).
@@ -99,7 +103,7 @@ struct Args {
   const RowCoef *coef;
   const StageSlot *slots;
   const FiltRow *frows;
-  const double *twid;          // [N][2]: cos, sin of 2 pi k / N
+  const double *twid;          // [N/2+1][2]: cos, sin of 2 pi k / N
   int fft_L;                   // complex FFT length: N/2 (even N) or N
   int fft_half;                // 1: even N, real FFT through a half-length complex FFT
   int fft_nrad;                // radix passes
@@ -109,6 +113,13 @@ struct Args {
   int m0;                      // first member of a row launch
   int nseg;                    // longitude segments of kSeg points per row
   int ntiles;                  // work items of this launch
+  // k_phase: nnear whole-row filter tiles (one per FiltRow of the active
+  // members; blocks 0..nnear_pad-1), then row tiles over the rows with a
+  // component outside the filter, given as nfar_rng ranges of far_cnt[k]
+  // rows from far_lo[k] (nfar_rows rows in all)
+  int nnear, nnear_pad;
+  int nfar_rng, nfar_rows;
+  int far_lo[kMaxFarRng], far_cnt[kMaxFarRng];
   int in_full;                 // k_diffuse: read stored u, v of s_in
   int nslots;
   int upd_wpm, nmem_launch;    // k_update: workgroups per member, members

@@ -1,7 +1,6 @@
 // gd_fft.hpp -- the polar zonal filter of one staged tendency row
 // (filter_mod.F90:105-172) fused with the rescale of space_operators
-// (dycore_mod.F90:210-217) and the phase epilogue; used by k_filter and by
-// the whole-row tiles of k_tend.
+// (dycore_mod.F90:210-217) and the phase epilogue; used by k_filter.
 //
 // FFTPACK5's rfft1f -> mask -> rfft1b of one row is the orthogonal projection
 // onto the Fourier modes the mask keeps: wavenumbers 0..c and the cosine part
@@ -16,10 +15,12 @@
 //           part, masked symmetrically and transformed back.
 // The complex FFT is a Stockham autosort FFT (out-of-place passes between two
 // LDS buffers, so every pass is one read, one write and one barrier) with
-// radix-8/4/2/3/5 butterflies written out with their exact constants, a
-// table-driven radix-7 butterfly, and a direct O(R^2) pass for other prime
-// factors.  Twiddles W_N^k come from an FP64 table (cos, sin of 2 pi k/N)
-// loaded into LDS.  Everything is FP64 in every precision mode.
+// radix-8/4/2/3/5 butterflies written out with their exact constants and a
+// direct O(R^2) pass for other prime factors (7 included: a dedicated
+// radix-7 butterfly needs about 130 VGPRs, which would cap every kernel that
+// holds the filter at three waves per SIMD).  Twiddles W_N^k come from an FP64 table of cos, sin of 2 pi k/N
+// for k = 0..N/2, loaded into LDS; W_N^k for k > N/2 is the conjugate of
+// W_N^(N-k).  Everything is FP64 in every precision mode.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,10 +38,18 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// W_N^m = exp(-2 pi i m/N) (forward) or its conjugate (inverse).
-__device__ __forceinline__ double2 twf(const double2 *tw, int m, bool inv) {
-  const double2 c = tw[m];
-  return make_double2(c.x, inv ? c.y : -c.y);
+// The twiddle table in LDS: t[k] = (cos, sin)(2 pi k/n) for k = 0..n/2.
+struct Tw {
+  const double2 *t;
+  int n;
+};
+
+// W_n^m = exp(-2 pi i m/n) (forward) or its conjugate (inverse), 0 <= m < n.
+__device__ __forceinline__ double2 twf(Tw tw, int m, bool inv) {
+  const bool hi = 2 * m > tw.n;
+  const double2 c = tw.t[hi ? tw.n - m : m];
+  const double s = hi ? -c.y : c.y;  // sin(2 pi m/n)
+  return make_double2(c.x, inv ? s : -s);
 }
 
 // i * z (inverse) or -i * z (forward)
@@ -51,7 +60,7 @@ __device__ __forceinline__ double2 rot(double2 z, bool inv) {
 // In-register DFT of R points.  Radix 2, 3, 4, 5 and 8 use their closed
 // forms; other radices use W_R = W_N^rstep from the table.
 template <int R>
-__device__ __forceinline__ void dft_small(double2 (&v)[R], const double2 *tw, int rstep, bool inv) {
+__device__ __forceinline__ void dft_small(double2 (&v)[R], Tw tw, int rstep, bool inv) {
   if constexpr (R == 2) {
     const double2 a = v[0], b = v[1];
     v[0] = cadd(a, b);
@@ -127,8 +136,8 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2 *tw, in
 // points j + q L/R, twiddles them by W_{Ns R}^{q (j mod Ns)} and writes its
 // outputs to ((j - j mod Ns) R + j mod Ns) + q Ns.
 template <int R>
-__device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L, int Ns,
-                                         const double2 *tw, int tstride, bool inv) {
+__device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L, int Ns, Tw tw,
+                                         int tstride, bool inv) {
   const unsigned Lr = L / R;
   const int tws = (L / (Ns * R)) * tstride;
   const int rstep = (int)Lr * tstride;
@@ -149,8 +158,8 @@ __device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L
 }
 
 // The same pass for any radix R (prime factors above 7), O(R^2) per butterfly.
-__device__ __forceinline__ void fft_pass_any(const double2 *src, double2 *dst, int L, int Ns, int R,
-                                             const double2 *tw, int tstride, bool inv) {
+__device__ __forceinline__ void fft_pass_any(const double2 *src, double2 *dst, int L, int Ns, int R, Tw tw,
+                                             int tstride, bool inv) {
   const int Lr = L / R;
   const int tws = (L / (Ns * R)) * tstride;
   const int rstep = Lr * tstride;
@@ -171,8 +180,7 @@ __device__ __forceinline__ void fft_pass_any(const double2 *src, double2 *dst, i
 
 // Runs the plan's passes starting from buffer `cur`; returns the buffer that
 // holds the result.  Ends with a barrier.
-__device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const Args &a,
-                                       const double2 *tw, bool inv) {
+__device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const Args &a, Tw tw, bool inv) {
   const int L = a.fft_L, tstride = a.fft_half ? 2 : 1;
   int Ns = 1;
   for (int p = 0; p < a.fft_nrad; ++p) {
@@ -184,7 +192,6 @@ __device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const 
       case 3: fft_pass<3>(src, dst, L, Ns, tw, tstride, inv); break;
       case 4: fft_pass<4>(src, dst, L, Ns, tw, tstride, inv); break;
       case 5: fft_pass<5>(src, dst, L, Ns, tw, tstride, inv); break;
-      case 7: fft_pass<7>(src, dst, L, Ns, tw, tstride, inv); break;
       case 8: fft_pass<8>(src, dst, L, Ns, tw, tstride, inv); break;
       default: fft_pass_any(src, dst, L, Ns, R, tw, tstride, inv); break;
     }
@@ -209,8 +216,7 @@ __device__ __forceinline__ double2 fmask(double2 X, int k, int c) {
 // E'_k + i O'_k with E'_k = (Y_k + conj Y_{L-k})/2, O'_k = (Y_k - conj
 // Y_{L-k}) W_N^{-k}/2, scaled by 1/L for the unnormalised inverse.  Pairs
 // (k, L-k) are handled together; k = 0 carries X_0 and the Nyquist X_L.
-__device__ __forceinline__ void real_mid(const double2 *s, double2 *d, int L, int c, const double2 *tw,
-                                         double sc) {
+__device__ __forceinline__ void real_mid(const double2 *s, double2 *d, int L, int c, Tw tw, double sc) {
   const int H = L / 2 + 1;
   for (int k = threadIdx.x; k < H; k += kBlock) {
     if (k == 0) {
@@ -246,25 +252,28 @@ __device__ __forceinline__ void cplx_mid(const double2 *s, double2 *d, int L, in
   }
 }
 
-// Filters one staged row (staging slot fr.slot of member mb) and applies
-// the rescale and the phase epilogue.  KP: longitudes per thread (N <= KP *
-// kBlock).  smem holds filter_lds_bytes(N, L) bytes and is free on entry
-// (callers reusing it put a barrier before the call); red holds 8 doubles.
-// Ends with a barrier.
-template <typename CT, int KP>
-__device__ __forceinline__ void filter_row(const Args &a, const Member &mb, const FiltRow &fr,
-                                           unsigned char *smem, double *red) {
+// Filters one tendency row and applies the rescale and the phase epilogue.
+// XIN = false: the row is staging slot fr.slot of member mb; the function
+// fills the twiddle table and the row buffer b0 itself.  XIN = true: the
+// caller has written the row to b0 in the FFT layout (even N: N reals;
+// odd N: N complex points with zero imaginary parts) and the twiddle table
+// to tw, followed by a barrier.  KP: longitudes per thread (N <= KP *
+// kBlock).  tw holds tw_entries(N) entries, b0 and b1 fft_L complex points
+// each; red holds 8 doubles.  Ends with a barrier.
+template <typename CT, int KP, bool XIN>
+__device__ __forceinline__ void filter_row(const Args &a, const Member &mb, const FiltRow &fr, double2 *tw,
+                                           double2 *b0, double2 *b1, double *red) {
   const int N = a.N, P = a.pitch, L = a.fft_L, tid = threadIdx.x;
-  double2 *tw = (double2 *)smem;
-  double2 *b0 = tw + N;
-  double2 *b1 = b0 + L;
   const bool rescale = a.phase <= PH_T;
   const double sc = 1.0 / (double)L;
   const int comp = fr.comp;
   const size_t o = (size_t)fr.row * P;
+  const int ystep = a.fft_half ? 1 : 2;
+  double *rowbuf = (double *)b0;
 
-  // the staged row, its rescale weight (state%iap%u, %v, gd + ghs) and the
-  // epilogue inputs (state s0 or Tb), all loads issued together
+  // the row's rescale weight (state%iap%u, %v, gd + ghs) and the epilogue
+  // inputs (state s0 or Tb), and for a staged row the row itself, all loads
+  // issued together
   const double *xp = mb.sx() + (size_t)fr.slot * P;
   const double *wp = mb.st(a.s_in, comp == CDU ? FU : (comp == CDV ? FV : FGD)) + o;
   const double *wq = comp == CDGD ? mb.ghs() + o : nullptr;
@@ -274,7 +283,7 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
     const int l = tid + k * kBlock;
     double xv = 0.0, wv = 0.0, pv = 0.0;
     if (l < N) {
-      xv = xp[l];
+      if (!XIN) xv = xp[l];
       if (rescale) {
         wv = wq ? wp[l] + wq[l] : wp[l];
         pv = phase_pre<CT>(a, mb, comp, o + l);
@@ -284,9 +293,8 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
     w[k] = wv;
     pre[k] = pv;
   }
-  for (int k = tid; k < N; k += kBlock) tw[k] = make_double2(a.twid[2 * k], a.twid[2 * k + 1]);
-  {
-    double *rowbuf = (double *)b0;
+  if (!XIN) {
+    for (int k = tid; k < tw_entries(N); k += kBlock) tw[k] = make_double2(a.twid[2 * k], a.twid[2 * k + 1]);
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       const int l = tid + k * kBlock;
@@ -298,18 +306,26 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
         rowbuf[2 * l + 1] = 0.0;
       }
     }
+    __syncthreads();
+  } else {
+    // the row values are read before the first pass; b0 is first written
+    // by the second pass, after a barrier
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int l = tid + k * kBlock;
+      x[k] = l < N ? rowbuf[(size_t)l * ystep] : 0.0;
+    }
   }
-  __syncthreads();
 
-  int cur = fft_run(b0, b1, 0, a, tw, false);
+  const Tw twt{tw, N};
+  int cur = fft_run(b0, b1, 0, a, twt, false);
   if (a.fft_half)
-    real_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, tw, sc);
+    real_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, twt, sc);
   else
     cplx_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, sc);
   __syncthreads();
-  cur = fft_run(b0, b1, cur ^ 1, a, tw, true);
+  cur = fft_run(b0, b1, cur ^ 1, a, twt, true);
   const double *y = (const double *)(cur ? b1 : b0);
-  const int ystep = a.fft_half ? 1 : 2;
 
   // s1 = sum(tend * weight) before and s2 = sum(filtered tend * weight)
   // after filtering (dycore_mod.F90:211, 214), reduced together
@@ -370,6 +386,6 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
   __syncthreads();  // LDS free for the caller
 }
 
-inline size_t filter_lds_bytes_h(int N, int L) { return (size_t)N * 16 + (size_t)2 * L * 16; }
+inline size_t filter_lds_bytes_h(int N, int L) { return (size_t)tw_entries(N) * 16 + (size_t)2 * L * 16; }
 
 }  // namespace gd

@@ -1,6 +1,6 @@
 // gd_filter.hip -- k_filter: the polar filter of every staged row as its own
-// workgroup (filter_row in gd_fft.hpp).  Used where the rows are not filtered
-// inside k_tend: ordinary_diffusion's tendencies and the filter_rows seam.
+// workgroup (filter_row in gd_fft.hpp), after a k_tend or k_diffuse launch
+// that staged the rows.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -20,7 +20,9 @@ __global__ void __launch_bounds__(kBlock) k_filter(const Args a) {
   const Member mb = member_at(a, fr.member);
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[4 * 2];
-  filter_row<CT, KP>(a, mb, fr, smem, red);
+  double2 *tw = (double2 *)smem;
+  double2 *b0 = tw + tw_entries(a.N);
+  filter_row<CT, KP, false>(a, mb, fr, tw, b0, b0 + a.fft_L, red);
 }
 
 size_t filter_lds_bytes(int N, int L) { return filter_lds_bytes_h(N, L); }

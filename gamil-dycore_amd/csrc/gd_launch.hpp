@@ -36,5 +36,10 @@ hipError_t configure_kernels(int max_lds);
 // polar filter of the staged rows + rescale + phase epilogue (gd_filter.hip)
 hipError_t launch_filter(const Args &a, int prec, hipStream_t s);
 hipError_t configure_filter(int max_lds);
+// one tendency phase with its filter rows in one launch (gd_phase.hip);
+// N <= 4 * kBlock
+hipError_t launch_phase(const Args &a, int prec, int pass, bool in_full, hipStream_t s);
+size_t phase_lds_bytes(int N, int L, int prec);
+hipError_t configure_phase(int max_lds);
 
 }  // namespace gd

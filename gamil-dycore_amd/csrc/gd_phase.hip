@@ -1,0 +1,175 @@
+// gd_phase.hip -- k_phase: one tendency phase of the integrator (the
+// tendency, the polar filter of its flagged rows, the rescale and the phase
+// epilogue) in a single launch, in place of k_tend followed by k_filter.
+//
+// Blocks 0..nnear_pad-1 are whole-row filter tiles, one per flagged (member,
+// row, component).  Such a tile evaluates its component's tendency over the
+// whole row, segment by segment, into an LDS row buffer (tend_segment, the
+// arithmetic of the row tiles), then filters that buffer in place
+// (filter_row).  These are the longest chains of the launch, so they take the
+// lowest block indices: they are dispatched first, round-robin over the
+// XCDs.  The remaining blocks are the row tiles of k_tend over the rows that
+// have a component outside the filter.  Every value is formed by the same
+// expressions in the same order as with k_tend + k_filter, so both paths give
+// the same bits.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gd_common.hpp"
+#include "gd_device.hpp"
+#include "gd_fft.hpp"
+#include "gd_launch.hpp"
+#include "gd_tend.hpp"
+
+namespace gd {
+
+// LDS: [stencil rows of a segment, later FFT buffer b1] [row buffer b0]
+// [twiddle table].
+template <typename CT>
+__host__ __device__ inline size_t phase_region0(int L) {
+  const size_t st = tend_stencil_bytes<CT>(), fb = (size_t)L * 16;
+  return align16(st > fb ? st : fb);
+}
+
+template <typename CT>
+__host__ __device__ inline size_t phase_lds_t(int N, int L) {
+  return phase_region0<CT>(L) + (size_t)L * 16 + (size_t)tw_entries(N) * 16;
+}
+
+// Row value x of longitude l into the FFT layout of b0.
+__device__ __forceinline__ void row_put(const Args &a, double *rowbuf, int l, double x) {
+  if (a.fft_half) {
+    rowbuf[l] = x;
+  } else {
+    rowbuf[2 * l] = x;
+    rowbuf[2 * l + 1] = 0.0;
+  }
+}
+
+// Component CM of full row j (half row j for dv), segment by segment; the
+// loads of segment g+1 are issued before segment g is evaluated, so only the
+// first segment waits a full load round.
+template <typename CT, int PASS, bool IN_FULL, int CM>
+__device__ __forceinline__ void near_row(const Args &a, const Member &mb, const RowCoef &cj, int j, double *rowbuf,
+                                         unsigned char *smem, double *red) {
+  const int N = a.N, tid = threadIdx.x;
+  SegLoads cur;
+  seg_load<PASS, IN_FULL>(a, mb, j, 0, min(N, kSeg), cur);
+  for (int g = 0; g < a.nseg; ++g) {
+    const int i0 = g * kSeg, W = min(N - i0, kSeg);
+    SegLoads nxt;
+    if (g + 1 < a.nseg) seg_load<PASS, IN_FULL>(a, mb, j, i0 + kSeg, min(N - i0 - kSeg, kSeg), nxt);
+    if (g) __syncthreads();  // the previous segment's stencil rows have been read
+    const TendPoint pt = seg_eval<CT, PASS, IN_FULL, CM>(a, cj, j, i0, W, cur, false, 0.0,
+                                                         TendPoint{0, 0, 0, 0, 0, 0}, smem, red);
+    if (tid < W) row_put(a, rowbuf, i0 + tid, CM == TC_DU ? pt.du : (CM == TC_DV ? pt.dv : pt.dgd));
+    cur = nxt;
+  }
+}
+
+template <typename CT, int PASS, bool IN_FULL, int KP>
+__device__ __forceinline__ void near_tile(const Args &a, const FiltRow &fr, unsigned char *smem, double *red) {
+  const int N = a.N, L = a.fft_L, tid = threadIdx.x;
+  const Member mb = member_at(a, fr.member);
+  const int j = fr.row;
+  const RowCoef cj = a.coef[j];
+  double2 *b1 = (double2 *)smem;
+  double2 *b0 = (double2 *)(smem + phase_region0<CT>(L));
+  double2 *tw = b0 + L;
+  // the twiddle loads are issued first and reach LDS after the tendency
+  // loop, so their latency overlaps the first segment's
+  constexpr int TK = KP / 2 + 1;  // tw_entries(KP * kBlock) <= TK * kBlock
+  const int nt = tw_entries(N);
+  double2 twr[TK];
+#pragma unroll
+  for (int k = 0; k < TK; ++k) {
+    const int e = tid + k * kBlock;
+    twr[k] = e < nt ? make_double2(a.twid[2 * e], a.twid[2 * e + 1]) : make_double2(0.0, 0.0);
+  }
+  double *rowbuf = (double *)b0;
+  if (fr.comp == CDU) {
+    near_row<CT, PASS, IN_FULL, TC_DU>(a, mb, cj, j, rowbuf, smem, red);
+  } else if (fr.comp == CDV) {
+    near_row<CT, PASS, IN_FULL, TC_DV>(a, mb, cj, j, rowbuf, smem, red);
+  } else if (PASS != 2 && (j == 0 || j == a.M - 1)) {
+    // a pole row's dgd is minus the pole flux at every longitude
+    // (dycore_mod.F90:572-596)
+    const double pole = pole_flux(a, mb, j, red);
+    for (int l = tid; l < N; l += kBlock) row_put(a, rowbuf, l, -pole);
+  } else if (PASS == 2) {
+    // the slow pass has no mass continuity term
+    for (int l = tid; l < N; l += kBlock) row_put(a, rowbuf, l, 0.0);
+  } else {
+    near_row<CT, PASS, IN_FULL, TC_DGD>(a, mb, cj, j, rowbuf, smem, red);
+  }
+#pragma unroll
+  for (int k = 0; k < TK; ++k) {
+    const int e = tid + k * kBlock;
+    if (e < nt) tw[e] = twr[k];
+  }
+  __syncthreads();  // row and twiddles published; the stencil rows are free for b1
+  filter_row<CT, KP, true>(a, mb, fr, tw, b0, b1, red);
+}
+
+// Five waves per SIMD (at most 96 VGPRs): with the LDS of an FP64 tile that
+// is five tiles per CU, so a 720x361 run's phase fits in one round of
+// resident workgroups.
+template <typename CT, int PASS, bool IN_FULL, int KP>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) k_phase(const Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[4 * 6];
+  const int b = blockIdx.x;
+  if (b < a.nnear_pad) {
+    if (b < a.nnear) near_tile<CT, PASS, IN_FULL, KP>(a, a.frows[b], smem, red);
+    return;
+  }
+  // row tiles: each XCD owns a contiguous band of them (nnear_pad and the
+  // remaining grid are multiples of kXcd, so block b stays on XCD b % kXcd)
+  const int bb = b - a.nnear_pad, per = (gridDim.x - a.nnear_pad) / kXcd;
+  const int t = (bb % kXcd) * per + bb / kXcd;
+  if (t >= a.ntiles) return;
+  tend_tile<CT, PASS, IN_FULL, true>(a, t, smem, red);
+}
+
+size_t phase_lds_bytes(int N, int L, int prec) {
+  return prec == 0 ? phase_lds_t<double>(N, L) : phase_lds_t<float>(N, L);
+}
+
+template <typename CT, int KP>
+static hipError_t launch_phase_k(const Args &a, int pass, bool in_full, hipStream_t s) {
+  const int rows = ((a.ntiles + kXcd - 1) / kXcd) * kXcd;
+  dim3 g(a.nnear_pad + rows), b(kBlock);
+  const size_t lds = phase_lds_t<CT>(a.N, a.fft_L);
+#define GD_PH(P, F) hipLaunchKernelGGL((k_phase<CT, P, F, KP>), g, b, lds, s, a)
+  if (pass == 0) { if (in_full) GD_PH(0, true); else GD_PH(0, false); }
+  else if (pass == 1) { if (in_full) GD_PH(1, true); else GD_PH(1, false); }
+  else { if (in_full) GD_PH(2, true); else GD_PH(2, false); }
+#undef GD_PH
+  return hipGetLastError();
+}
+
+hipError_t launch_phase(const Args &a, int prec, int pass, bool in_full, hipStream_t s) {
+  if (a.N > 4 * kBlock) return hipErrorInvalidValue;
+  return prec == 0 ? launch_phase_k<double, 4>(a, pass, in_full, s) : launch_phase_k<float, 4>(a, pass, in_full, s);
+}
+
+template <typename CT, int KP>
+static hipError_t configure_phase_k(int max_lds) {
+  const void *fns[] = {(const void *)k_phase<CT, 0, true, KP>,  (const void *)k_phase<CT, 0, false, KP>,
+                       (const void *)k_phase<CT, 1, true, KP>,  (const void *)k_phase<CT, 1, false, KP>,
+                       (const void *)k_phase<CT, 2, true, KP>,  (const void *)k_phase<CT, 2, false, KP>};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t configure_phase(int max_lds) {
+  hipError_t e;
+  if ((e = configure_phase_k<double, 4>(max_lds)) != hipSuccess) return e;
+  return configure_phase_k<float, 4>(max_lds);
+}
+
+}  // namespace gd
