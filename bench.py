@@ -44,11 +44,12 @@ def tend_launch_bytes(N, M, prec):
     return state + 8 * full + state + state + tend
 
 
-def traffic_from_profile():
+def traffic_from_profile(kernel):
     """HBM bytes per launch of the same kernel launch measured with
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a 256 MiB copy), as
-    committed in profiles/; None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_tend_720x361_traffic.txt")
+    committed in profiles/<kernel>_720x361_traffic.txt; None when the
+    summary is absent."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_720x361_traffic.txt")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -87,10 +88,11 @@ def run_gpu(gd, prec, ic, steps, warmup, repeats):
         d.step(int(full))
         lat.append(time.perf_counter() - s0)
     prof = d.profile_kernels(60)
-    tend_us = d.time_kernel("tend", 200)
+    fused = d.fused_phases()
+    kern_us = d.time_kernel("phase" if fused else "tend", 200)
     d.close()
     return dict(elapsed=elapsed, mass=mass[:, 0], energy=energy[:, 0], lat=lat, prof=prof, cut=cut,
-                tend_us=tend_us)
+                kern_us=kern_us, fused=fused)
 
 
 def run_batched(gd, prec, ic, members, steps):
@@ -150,12 +152,20 @@ def _time_steps(d, steps, warmup):
     return time.perf_counter() - t0
 
 
-def _tend_roofline(N, M, members, us):
-    """HBM roofline of the fast-pass second-predictor tendency launch over
-    all members (its algorithmic bytes over its replayed duration)."""
+def _kernel_us(d, reps):
+    """Replayed duration of the fast-pass second-predictor phase launch:
+    k_phase when the handle fuses phases, else k_tend."""
+    fused = d.fused_phases()
+    return d.time_kernel("phase" if fused else "tend", reps), fused
+
+
+def _tend_roofline(N, M, members, us, fused):
+    """HBM roofline of the fast-pass second-predictor phase launch over all
+    members (its algorithmic bytes over its replayed duration)."""
     nbytes = tend_launch_bytes(N, M, "mixed") * members
     achieved = nbytes / (us * 1e-6) / 1e9
-    return {"bound": "hbm", "kernel": "k_tend<float, fast pass> (second predictor)",
+    kname = "k_phase" if fused else "k_tend"
+    return {"bound": "hbm", "kernel": f"{kname}<float, fast pass> (second predictor)",
             "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": None, "bytes_per_launch": int(nbytes),
             "avg_launch_us": round(us, 3)}
@@ -178,10 +188,10 @@ def other_workload(gd, args):
             for k, ic in enumerate(ics):
                 d.set_state(*ic, member=k)
             el = _time_steps(d, steps, args.warmup)
-            tend_us[prec] = d.time_kernel("tend", 50)
+            tend_us[prec] = _kernel_us(d, 50)
             d.close()
             out[prec] = mem * steps * dt / 86400.0 / el
-        base["roofline"] = _tend_roofline(N, M, mem, tend_us["mixed"])
+        base["roofline"] = _tend_roofline(N, M, mem, *tend_us["mixed"])
         return dict(base, metric="member-days/sec, 64-member ensemble at 360x181 (mixed vs FP64)",
                     value=round(out["mixed"], 3), unit="member-days/s", steps=steps,
                     ms_per_step=round(mem * steps * dt / 86400.0 / out["mixed"] / steps * 1e3, 4),
@@ -199,10 +209,10 @@ def other_workload(gd, args):
             d = _cfg(gd, N, M, dt, 6.25e3, prec)
             d.set_state(*ic)
             el = _time_steps(d, steps, args.warmup)
-            tend_us[prec] = d.time_kernel("tend", 100)
+            tend_us[prec] = _kernel_us(d, 100)
             d.close()
             out[prec] = steps * dt / 86400.0 / el
-        base["roofline"] = _tend_roofline(N, M, 1, tend_us["mixed"])
+        base["roofline"] = _tend_roofline(N, M, 1, *tend_us["mixed"])
         return dict(base, metric="model-days/sec at 1440x721 (mixed vs FP64)",
                     value=round(out["mixed"], 4), unit="model-days/s", steps=steps,
                     ms_per_step=round(dt / 86400.0 / out["mixed"] * 1e3, 4), scaling="weak",
@@ -344,19 +354,24 @@ def main():
     days = args.steps * dt / 86400.0
     val = {p: days / r["elapsed"] for p, r in res.items()}
     mixed = res["mixed"]
-    # roofline of the tendency kernel: one fast-pass second-predictor launch,
-    # its algorithmic bytes over its duration replayed back to back (HIP
-    # events on the library's stream)
+    # roofline of the dominant kernel: one fast-pass second-predictor phase
+    # launch (k_phase: tendency, filter rows and epilogue in one launch; or
+    # k_tend when phases are not fused), its algorithmic bytes over its
+    # duration replayed back to back (HIP events on the library's stream).
+    # Fusing moves no extra bytes: the filter rows stay in LDS.
     prof = mixed["prof"]
     dom = max(prof, key=lambda k: prof[k][1])
     nbytes = tend_launch_bytes(N, M, "mixed")
-    avg_s = mixed["tend_us"] * 1e-6
+    avg_s = mixed["kern_us"] * 1e-6
     achieved = nbytes / avg_s / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_tend<float, fast pass> (second predictor)",
+    kname = "k_phase" if mixed["fused"] else "k_tend"
+    roofline = {"bound": "hbm",
+                "kernel": f"{kname}<float, fast pass> (second predictor" +
+                          (": tendency + filter rows + epilogue)" if mixed["fused"] else ")"),
                 "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 4), "traffic": traffic_from_profile(),
+                "frac": round(achieved / 8000.0, 4), "traffic": traffic_from_profile(kname),
                 "bytes_per_launch": int(nbytes),
-                "avg_launch_us": round(mixed["tend_us"], 3),
+                "avg_launch_us": round(mixed["kern_us"], 3),
                 "dominant_family_by_time": dom,
                 "family_ms_per_step": {k: round(v[1] / 60.0, 4) for k, v in prof.items()}}
     cpu = None

@@ -93,7 +93,7 @@ EXPORTS = (
     "gd_device_bytes", "gd_cutoff_from_lat0", "gd_time_kernel", "gd_history_write",
     "gd_restart_write", "gd_restart_read", "gd_io_last_error", "gd_ic_steady_geostrophic",
     "gd_ic_jet_zonal_flow", "gd_ic_shallow_water_waves", "gd_serve", "gd_serve_last_error",
-    "gd_set_active_members", "gd_copy_member",
+    "gd_set_active_members", "gd_copy_member", "gd_fused_phases",
 )
 
 _lib = None
@@ -148,6 +148,7 @@ def lib():
     L.gd_io_last_error.argtypes = []
     L.gd_set_active_members.argtypes = [vp, C.c_int32]
     L.gd_copy_member.argtypes = [vp, C.c_int32, C.c_int32]
+    L.gd_fused_phases.argtypes = [vp]
     L.gd_serve.argtypes = [C.POINTER(ServeConfig), C.POINTER(ServeRequest), C.c_int32,
                            C.POINTER(ServeResult), dp]
     L.gd_serve_last_error.restype = C.c_char_p
@@ -466,13 +467,18 @@ class Dycore:
         u, v, gd = self.get_state(member)
         write_restart(path, u, v, gd, self._ghs[member], **info)
 
+    def fused_phases(self):
+        """True when each tendency phase runs as one fused launch."""
+        return bool(lib().gd_fused_phases(self.h))
+
     def device_bytes(self):
         return int(lib().gd_device_bytes(self.h))
 
     def time_kernel(self, kernel, reps=200):
         """Microseconds per launch of one kernel replayed back to back:
-        kernel 'tend' (fast pass, second predictor), 'filter' or 'update'."""
+        kernel 'tend' (fast pass, second predictor), 'filter', 'update' or
+        'phase' (that phase as one fused launch; fused handles only)."""
         us = C.c_double(0.0)
-        kid = {"tend": 0, "filter": 1, "update": 2}[kernel]
+        kid = {"tend": 0, "filter": 1, "update": 2, "phase": 3}[kernel]
         _check(lib().gd_time_kernel(self.h, kid, int(reps), C.byref(us)), self.h)
         return us.value

@@ -52,8 +52,10 @@ struct gd_handle {
   int fft_L = 1, fft_half = 1;
   std::vector<int> fft_rad;
   // fused phase launches (k_phase): the filter rows inside the tendency
-  // launch; the row tiles cover the row ranges far_lo/far_cnt
-  bool fused = false;
+  // launch; the row tiles cover the row ranges far_lo/far_cnt.  fused_ok:
+  // the shape allows them; fused_mode: -1 by launch size, 0 never, 1 always
+  bool fused_ok = false;
+  int fused_mode = -1;
   int nfar_rng = 0, nfar_rows = 0;
   int far_lo[kMaxFarRng] = {}, far_cnt[kMaxFarRng] = {};
   int32_t *dstep = nullptr;
@@ -160,16 +162,10 @@ hipError_t enqueue_filter(gd_handle *h, Args a) {
   return run(h, "filter", [&] { return launch_filter(a, h->prec, h->stream); });
 }
 
-// One tendency phase (a.phase) of the active members with its filter rows:
-// one k_phase launch, or k_tend then k_filter.
-hipError_t enqueue_phase(gd_handle *h, Args a, int pass, bool in_full) {
-  a.ntiles = h->nact * h->M * a.nseg;
-  if (!h->fused) {
-    hipError_t e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, in_full, h->stream); });
-    if (e != hipSuccess) return e;
-    return enqueue_filter(h, a);
-  }
-  a.nnear = (int)(h->frows.size() / h->nmem) * h->nact;
+// The k_phase tiles of members 0..nm-1: their filter rows, then the row
+// tiles of the far-row ranges.
+void set_fused(const gd_handle *h, Args &a, int nm) {
+  a.nnear = (int)(h->frows.size() / h->nmem) * nm;
   a.nnear_pad = ((a.nnear + kXcd - 1) / kXcd) * kXcd;
   a.nfar_rng = h->nfar_rng;
   a.nfar_rows = h->nfar_rows;
@@ -177,7 +173,32 @@ hipError_t enqueue_phase(gd_handle *h, Args a, int pass, bool in_full) {
     a.far_lo[k] = h->far_lo[k];
     a.far_cnt[k] = h->far_cnt[k];
   }
-  a.ntiles = h->nact * h->nfar_rows * a.nseg;
+  a.ntiles = nm * h->nfar_rows * a.nseg;
+}
+
+// Fused phase launches pay off while a launch is a few rounds of resident
+// tiles (latency bound); past about eight rounds the lower occupancy of
+// k_phase (five tiles per CU against seven or eight for k_tend) costs more
+// than the launches it saves.  Measured crossovers on MI355X: 8 to 16 runs
+// of 720x361, 16 to 32 of 360x181, 32 to 128 of 180x91.
+constexpr int64_t kFusedMaxTiles = 10240;
+
+bool use_fused(const gd_handle *h) {
+  if (!h->fused_ok || h->fused_mode == 0) return false;
+  const int64_t tiles = (int64_t)h->nact * h->M * ((h->N + kSeg - 1) / kSeg);
+  return h->fused_mode == 1 || tiles <= kFusedMaxTiles;
+}
+
+// One tendency phase (a.phase) of the active members with its filter rows:
+// one k_phase launch, or k_tend then k_filter.
+hipError_t enqueue_phase(gd_handle *h, Args a, int pass, bool in_full) {
+  a.ntiles = h->nact * h->M * a.nseg;
+  if (!use_fused(h)) {
+    hipError_t e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, in_full, h->stream); });
+    if (e != hipSuccess) return e;
+    return enqueue_filter(h, a);
+  }
+  set_fused(h, a, h->nact);
   return run(h, "phase", [&] { return launch_phase(a, h->prec, pass, in_full, h->stream); });
 }
 
@@ -535,7 +556,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   // fused phase launches: every row with a component outside the filter
   // (and the pole rows, whose du = 0 goes through the row tiles' epilogue)
   // in at most kMaxFarRng contiguous ranges.  GD_FUSED=0 keeps k_tend +
-  // k_filter.
+  // k_filter, GD_FUSED=1 fuses at any launch size.
   {
     std::vector<int> far;
     for (int j = 0; j < h->M; ++j) {
@@ -558,9 +579,9 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
     h->nfar_rng = nr;
     h->nfar_rows = (int)far.size();
     const char *env = std::getenv("GD_FUSED");
-    const bool off = env && env[0] == '0';
-    h->fused = !off && fits && !h->frows.empty() && h->N <= 4 * kBlock &&
-               phase_lds_bytes(h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
+    h->fused_mode = env && env[0] == '0' ? 0 : (env && env[0] == '1' ? 1 : -1);
+    h->fused_ok = fits && !h->frows.empty() && h->N <= 4 * kBlock &&
+                  phase_lds_bytes(h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
   }
   configure_kernels((int)prop.sharedMemPerBlock);
   hipGetLastError();
@@ -876,9 +897,13 @@ int gd_copy_member(gd_handle *h, int32_t dst, int32_t src) {
   return GD_OK;
 }
 
+int gd_fused_phases(const gd_handle *h) { return h && use_fused(h) ? 1 : 0; }
+
 int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us) {
-  if (!h || reps < 1 || !us || kernel < GD_TIME_TEND || kernel > GD_TIME_UPDATE) return GD_ERR_INVALID_ARG;
+  if (!h || reps < 1 || !us || kernel < GD_TIME_TEND || kernel > GD_TIME_PHASE) return GD_ERR_INVALID_ARG;
   if (kernel == GD_TIME_FILTER && h->frows.empty()) return fail(h, GD_ERR_INVALID_ARG, "no filtered rows");
+  if (kernel == GD_TIME_PHASE && !use_fused(h))
+    return fail(h, GD_ERR_NOT_SUPPORTED, "the handle does not use fused phase launches");
   hipSetDevice(h->device);
   // arguments of the second predictor of a fast-pass call (predict_correct,
   // dycore_mod.F90:774-777): S2 = S0 + dt/2 L(S1)
@@ -908,6 +933,11 @@ int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us) {
       Args f = a;
       f.ntiles = (int)h->frows.size();
       return launch_filter(f, h->prec, h->stream);
+    }
+    if (kernel == GD_TIME_PHASE) {
+      Args f = a;
+      set_fused(h, f, h->nmem);
+      return launch_phase(f, h->prec, pass, false, h->stream);
     }
     return launch_update(a, h->prec, h->stream);
   };

@@ -343,8 +343,19 @@ int64_t gd_device_bytes(const gd_handle *h);
  * one pair of HIP events (inputs are whatever the state slots hold; the
  * state X is not modified).  kernel: GD_TIME_TEND (tendency kernel, fast
  * pass, second predictor), GD_TIME_FILTER (the filter of that phase),
- * GD_TIME_UPDATE (beta + corrector). */
-enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2 };
+ * GD_TIME_UPDATE (beta + corrector), GD_TIME_PHASE (that phase as one fused
+ * launch: tendency, filter rows and epilogue; GD_ERR_NOT_SUPPORTED when the
+ * handle steps with separate tendency and filter launches).  No counterpart
+ * in the reference: a measurement hook. */
+enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2, GD_TIME_PHASE = 3 };
+
+/* 1 when the handle's step, at its current active-member count, runs each
+ * tendency phase as one fused launch, else 0.  Phases are fused for
+ * num_lon <= 1024 with filtered rows while active members x num_lat x
+ * ceil(num_lon/254) <= 10240 row tiles; GD_FUSED=0 or 1 in the environment
+ * when the handle is created forces separate or fused launches.  No
+ * counterpart in the reference. */
+int gd_fused_phases(const gd_handle *h);
 int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us_per_launch);
 
 #ifdef __cplusplus
