@@ -432,12 +432,19 @@ void build_tables(gd_handle *h, std::vector<RowCoef> &rc) {
     if (g.half_flag[j]) add(CDV, j, g.half_cut[j]);
   for (int j = 0; j < M; ++j)
     if (g.full_flag[j]) add(CDGD, j, g.full_cut[j]);
-  // filter workgroups: one per staged row of each member.  A row's
-  // arithmetic does not depend on any other row, so a member gives the same
-  // bits alone and inside an ensemble.
+  // filter workgroups: one per staged row of each member, in (row,
+  // component) order so that k_phase can give the components of a row and
+  // its neighbours to one XCD.  A row's arithmetic does not depend on any
+  // other row, so a member gives the same bits alone and inside an ensemble.
+  std::vector<int> order(h->slots.size());
+  for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return h->slots[x].row != h->slots[y].row ? h->slots[x].row < h->slots[y].row
+                                              : h->slots[x].comp < h->slots[y].comp;
+  });
   h->frows.clear();
   for (int m = 0; m < h->nmem; ++m)
-    for (int sl = 0; sl < (int)h->slots.size(); ++sl) {
+    for (int sl : order) {
       const StageSlot &ss = h->slots[sl];
       FiltRow f{};
       f.member = m;

@@ -4,14 +4,13 @@
 //
 // Blocks 0..nnear_pad-1 are whole-row filter tiles, one per flagged (member,
 // row, component).  Such a tile evaluates its component's tendency over the
-// whole row, segment by segment, into an LDS row buffer (tend_segment, the
-// arithmetic of the row tiles), then filters that buffer in place
-// (filter_row).  These are the longest chains of the launch, so they take the
-// lowest block indices: they are dispatched first, round-robin over the
-// XCDs.  The remaining blocks are the row tiles of k_tend over the rows that
-// have a component outside the filter.  Every value is formed by the same
-// expressions in the same order as with k_tend + k_filter, so both paths give
-// the same bits.
+// whole row, segment by segment, into an LDS row buffer (the arithmetic of
+// the row tiles), then filters that buffer in place (filter_row).  These are
+// the longest chains of the launch, so they take the lowest block indices
+// and are dispatched first.  The remaining blocks are the row tiles of k_tend
+// over the rows that have a component outside the filter.  Every value is
+// formed by the same expressions in the same order as with k_tend +
+// k_filter, so both paths give the same bits.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -121,7 +120,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 
   __shared__ double red[4 * 6];
   const int b = blockIdx.x;
   if (b < a.nnear_pad) {
-    if (b < a.nnear) near_tile<CT, PASS, IN_FULL, KP>(a, a.frows[b], smem, red);
+    // filter rows are in (member, row, component) order; each XCD takes a
+    // contiguous run of them, so the components of a row and its
+    // neighbours share that XCD's L2
+    const int t = (b % kXcd) * (a.nnear_pad / kXcd) + b / kXcd;
+    if (t < a.nnear) near_tile<CT, PASS, IN_FULL, KP>(a, a.frows[t], smem, red);
     return;
   }
   // row tiles: each XCD owns a contiguous band of them (nnear_pad and the
