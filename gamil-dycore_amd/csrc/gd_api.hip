@@ -124,10 +124,12 @@ Args base_args(const gd_handle *h) {
   a.nseg = (h->N + kSeg - 1) / kSeg;
   a.ntiles = h->nact * h->M * a.nseg;  // (member, row, segment) tiles
   a.nslots = (int)h->slots.size();
-  // k_update: one row per workgroup for a single run; ensembles share about
-  // 512 workgroups (each forms its member's beta from the partials)
+  // k_update: two rows per workgroup for a single run (each workgroup forms
+  // its member's beta from all the partials, so fewer workgroups read less;
+  // one row each or four rows each measured slower at 720x361); ensembles
+  // share about 512 workgroups
   a.nmem_launch = h->nact;
-  a.upd_wpm = std::max(1, std::min(h->M, 512 / h->nact));
+  a.upd_wpm = std::max(1, std::min((h->M + 1) / 2, 512 / h->nact));
   a.upwind = h->cfg.uv_adv_scheme == GD_ADV_UPWIND;
   a.beta_lon = h->cfg.uv_adv_upwind_lon_beta;
   a.beta_lat = h->cfg.uv_adv_upwind_lat_beta;

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   const double *tcu = mb.tc(CDU), *tcv = mb.tc(CDV);
   const double *rcu = mb.st(SP2, Fu), *rcv = mb.st(SP2, Fv);
   double *og = mb.st(a.s_out, FGD), *ou = mb.st(a.s_out, FU), *ov = mb.st(a.s_out, FV);
-  constexpr int U = 4;  // points per thread per batch
+  constexpr int U = 8;  // points per thread per batch (two 720-point rows in one)
   size_t off[U];
   bool on[U], hv[U];
   double g0[U], u0[U], v0[U], tg[U], tu[U], tv[U];
