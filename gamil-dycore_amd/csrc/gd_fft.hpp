@@ -202,6 +202,29 @@ __device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const 
   return cur;
 }
 
+// The same passes with the plan fixed at compile time: L, the stride and
+// every Ns are constants, so the index arithmetic of each pass (j mod Ns,
+// the twiddle strides) folds to shifts and constant multiplies.  The
+// arithmetic on the data is that of fft_run, so both give the same bits.
+template <int L, int TS, int NS, int R, int... REST>
+__device__ __forceinline__ int fft_fixed(double2 *b0, double2 *b1, int cur, Tw tw, bool inv) {
+  fft_pass<R>(cur ? b1 : b0, cur ? b0 : b1, L, NS, tw, TS, inv);
+  __syncthreads();
+  if constexpr (sizeof...(REST) > 0)
+    return fft_fixed<L, TS, NS * R, REST...>(b0, b1, cur ^ 1, tw, inv);
+  else
+    return cur ^ 1;
+}
+
+// The plans of the even row lengths 720, 360 and 180 (fft_plan of their
+// half lengths) run fixed; others run through fft_run.
+__device__ __forceinline__ int fft_any(double2 *b0, double2 *b1, int cur, const Args &a, Tw tw, bool inv) {
+  if (a.fft_half && a.fft_L == 360) return fft_fixed<360, 2, 1, 8, 3, 3, 5>(b0, b1, cur, tw, inv);
+  if (a.fft_half && a.fft_L == 180) return fft_fixed<180, 2, 1, 4, 3, 3, 5>(b0, b1, cur, tw, inv);
+  if (a.fft_half && a.fft_L == 90) return fft_fixed<90, 2, 1, 2, 3, 3, 5>(b0, b1, cur, tw, inv);
+  return fft_run(b0, b1, cur, a, tw, inv);
+}
+
 // FFTPACK5 mask of cutoff c on the complex spectrum: wavenumbers 0..c kept,
 // only the cosine (real) part of wavenumber c+1 kept, the rest removed.
 __device__ __forceinline__ double2 fmask(double2 X, int k, int c) {
@@ -318,13 +341,13 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
   }
 
   const Tw twt{tw, N};
-  int cur = fft_run(b0, b1, 0, a, twt, false);
+  int cur = fft_any(b0, b1, 0, a, twt, false);
   if (a.fft_half)
     real_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, twt, sc);
   else
     cplx_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, sc);
   __syncthreads();
-  cur = fft_run(b0, b1, cur ^ 1, a, twt, true);
+  cur = fft_any(b0, b1, cur ^ 1, a, twt, true);
   const double *y = (const double *)(cur ? b1 : b0);
 
   // s1 = sum(tend * weight) before and s2 = sum(filtered tend * weight)
