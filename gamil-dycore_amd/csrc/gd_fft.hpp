@@ -296,24 +296,28 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
 
   // the row's rescale weight (state%iap%u, %v, gd + ghs) and the epilogue
   // inputs (state s0 or Tb), and for a staged row the row itself, all loads
-  // issued together
+  // issued together; gd + ghs is added after the forward transform, when
+  // the loads have long arrived
   const double *xp = mb.sx() + (size_t)fr.slot * P;
   const double *wp = mb.st(a.s_in, comp == CDU ? FU : (comp == CDV ? FV : FGD)) + o;
-  const double *wq = comp == CDGD ? mb.ghs() + o : nullptr;
-  double x[KP], w[KP], pre[KP];
+  const bool gdrow = comp == CDGD;
+  const double *wq = mb.ghs() + o;
+  double x[KP], w[KP], wg[KP], pre[KP];
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
     const int l = tid + k * kBlock;
-    double xv = 0.0, wv = 0.0, pv = 0.0;
+    double xv = 0.0, wv = 0.0, gv = 0.0, pv = 0.0;
     if (l < N) {
       if (!XIN) xv = xp[l];
       if (rescale) {
-        wv = wq ? wp[l] + wq[l] : wp[l];
+        wv = wp[l];
+        if (gdrow) gv = wq[l];
         pv = phase_pre<CT>(a, mb, comp, o + l);
       }
     }
     x[k] = xv;
     w[k] = wv;
+    wg[k] = gv;
     pre[k] = pv;
   }
   if (!XIN) {
@@ -342,6 +346,10 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
 
   const Tw twt{tw, N};
   int cur = fft_any(b0, b1, 0, a, twt, false);
+  if (gdrow) {
+#pragma unroll
+    for (int k = 0; k < KP; ++k) w[k] = w[k] + wg[k];
+  }
   if (a.fft_half)
     real_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, twt, sc);
   else

@@ -68,7 +68,7 @@ __device__ __forceinline__ void near_row(const Args &a, const Member &mb, const 
 }
 
 template <typename CT, int PASS, bool IN_FULL, int KP>
-__device__ __forceinline__ void near_tile(const Args &a, const FiltRow &fr, unsigned char *smem, double *red) {
+__device__ __forceinline__ void near_tile(const Args &a, const FiltRow fr, unsigned char *smem, double *red) {
   const int N = a.N, L = a.fft_L, tid = threadIdx.x;
   const Member mb = member_at(a, fr.member);
   const int j = fr.row;
@@ -124,6 +124,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 
     // contiguous run of them, so the components of a row and its
     // neighbours share that XCD's L2
     const int t = (b % kXcd) * (a.nnear_pad / kXcd) + b / kXcd;
+    // (the row descriptor is copied: a reference into global memory would be
+    // reloaded after every store of the epilogue)
     if (t < a.nnear) near_tile<CT, PASS, IN_FULL, KP>(a, a.frows[t], smem, red);
     return;
   }
