@@ -242,6 +242,12 @@ hipError_t enqueue_diffusion(gd_handle *h, int s_in, bool in_full, double dt) {
     a.s_lap = src >= 0 ? src : 0;
     a.lap_final = k == norder;
     a.s_lapout = src == SP1 ? SP2 : SP1;
+    if (a.lap_final && use_fused(h)) {
+      // the last application with its filter rows in one launch
+      Args f = a;
+      set_fused(h, f, h->nact);
+      return run(h, "diffuse", [&] { return launch_dphase(f, h->prec, h->stream); });
+    }
     e = run(h, "diffuse", [&] { return launch_diffuse(a, h->prec, h->stream); });
     if (e != hipSuccess) return e;
     src = a.s_lapout;

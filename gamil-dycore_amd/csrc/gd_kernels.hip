@@ -21,6 +21,7 @@
 #include "gd_common.hpp"
 #include "gd_launch.hpp"
 #include "gd_tend.hpp"
+#include "gd_diffuse.hpp"
 
 namespace gd {
 
@@ -241,171 +242,20 @@ __global__ void __launch_bounds__(kBlock) k_isp(const Args a) {
 // ordinary_diffusion (diffusion_mod.F90:74-217), order 2, FP64 in every mode
 // ---------------------------------------------------------------------------
 
-// Tile = (member, full row j, segment); all loads issued before the barrier
-// like k_tend.  LDS slot li = i - i0 + 2 holds longitude i.  One launch is
-// one application of the second-order operator (diffusion_mod.F90:107-179):
+// One tile per (member, row, segment) (dif_tile, gd_diffuse.hpp).  One launch
+// is one application of the second-order operator (diffusion_mod.F90:107-179):
 // diffusion_order/2 applications run in sequence, the first on the state
 // s_in (lap_src = 0), later ones on the previous result held in slot s_lap
-// (gd, u, v fields, lap_src = 1).  Intermediate applications write slot
-// s_lapout (lap_final = 0); the last stages the flagged rows for the filter
-// or adds sign*dt*nu times the result to the state s_in into X.
+// (lap_src = 1).  Intermediate applications write slot s_lapout
+// (lap_final = 0); the last stages the flagged rows for the filter and adds
+// sign*dt*nu times the result to the state s_in into X for the others.
 template <typename CT>
 __global__ void __launch_bounds__(kBlock) k_diffuse(const Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[16];
   const int t = xcd_tile(a.ntiles);
   if (t < 0) return;
-  const int N = a.N, M = a.M, P = a.pitch, nseg = a.nseg;
-  const int mloc = t / (M * nseg), rem = t - mloc * (M * nseg);
-  const int j = rem / nseg, g = rem - (rem / nseg) * nseg;
-  const Member mb = member_at(a, a.m0 + mloc);
-  const int i0 = g * kSeg, W = min(N - i0, kSeg);
-  constexpr int LW = kBlock + 4;
-  __shared__ double sgd[3][LW];  // gd rows j-1 .. j+1
-  __shared__ double su[3][LW];   // u rows j-1 .. j+1 (0 on and beyond the poles)
-  __shared__ double sv[3][LW];   // v half rows j-1 .. j+1 (0 outside)
-  __shared__ double red[16];
-  const int tid = threadIdx.x;
-  const double *gd = mb.st(a.s_in, FGD);
-  const double *Ug = mb.st(a.s_in, FU);
-  const double *Vg = mb.st(a.s_in, FV);
-  const double *ug = mb.st(a.s_in, Fu), *vg = mb.st(a.s_in, Fv);
-  // operand of this application: the state, or the previous result (raw gd,
-  // u, v, nothing derived)
-  const bool prev = a.lap_src != 0;
-  const bool full = a.in_full || prev;
-  const double *lgd = prev ? mb.st(a.s_lap, FGD) : gd;
-  const double *lu = prev ? mb.st(a.s_lap, Fu) : (a.in_full ? ug : Ug);
-  const double *lv = prev ? mb.st(a.s_lap, Fv) : (a.in_full ? vg : Vg);
-  auto wrap = [N](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
-  // thread e holds point i0-1+e (e < W+2 <= kBlock)
-  double lg[4], lgn[3], lU[3], lV[3];
-  {
-    const int e = tid;
-    const bool on = e < W + 2;
-    const int p = wrap(i0 - 1 + e), pn = wrap(i0 + e);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = j - 1 + r;
-      lg[r] = on && row >= 0 && row <= M - 1 ? lgd[(size_t)row * P + p] : 1.0;
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const int row = j - 1 + r;
-      const bool urow = on && row >= 1 && row <= M - 2;
-      const bool vrow = on && row >= 0 && row <= M - 2;
-      lgn[r] = urow && !full ? gd[(size_t)row * P + pn] : 1.0;
-      lU[r] = urow ? lu[(size_t)row * P + p] : 0.0;
-      lV[r] = vrow ? lv[(size_t)row * P + p] : 0.0;
-    }
-  }
-  // the state at the tile's own point for the final update when the operand
-  // is a previous result (u, v derived from U, V exactly as update_state)
-  double x_g = 0.0, x_u = 0.0, x_v = 0.0;
-  if (prev && a.lap_final && tid < W) {
-    const int i = i0 + tid, ip = i + 1 == N ? 0 : i + 1;
-    const size_t o = (size_t)j * P + i;
-    x_g = gd[o];
-    if (j >= 1 && j <= M - 2)
-      x_u = a.in_full ? ug[o] : Ug[o] * 2.0 / (sqrt(gd[o]) + sqrt(gd[o - i + ip]));
-    if (j <= M - 2) x_v = a.in_full ? vg[o] : Vg[o] * 2.0 / (sqrt(gd[o]) + sqrt(gd[o + P]));
-  }
-  // pole rows of gdd: zonal sums (diffusion_mod.F90:116-133)
-  const bool pole_row = j == 0 || j == M - 1;
-  double pacc = 0.0;
-  if (pole_row) {
-    const int ja = j == 0 ? 1 : M - 1, jb = j == 0 ? 0 : M - 2;
-#pragma unroll 4
-    for (int q = tid; q < N; q += kBlock) pacc += lgd[(size_t)ja * P + q] - lgd[(size_t)jb * P + q];
-  }
-  if (tid < W + 2) {
-    const int li = tid + 1;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const int row = j - 1 + r;
-      sgd[r][li] = lg[r];
-      double uv = 0.0, vv = 0.0;
-      if (row >= 1 && row <= M - 2)
-        uv = full ? lU[r] : lU[r] * 2.0 / (sqrt(lg[r]) + sqrt(lgn[r]));
-      if (row >= 0 && row <= M - 2)
-        vv = full ? lV[r] : lV[r] * 2.0 / (sqrt(lg[r]) + sqrt(lg[r + 1]));
-      su[r][li] = uv;
-      sv[r][li] = vv;
-    }
-  }
-  const RowCoef cj = a.coef[j];
-  const bool has_u = j >= 1 && j <= M - 2;
-  const bool has_v = j <= M - 2;
-  const double hc_m = j >= 1 ? a.coef[j - 1].hcos : 0.0;
-  const double hc_0 = has_v ? cj.hcos : 0.0;
-  const double fc_p = j + 1 <= M - 1 ? a.coef[j + 1].fcos : 0.0;
-  double pole = 0.0;
-  if (pole_row) {
-    const double s = block_sum(pacc, red);  // its barriers also publish the LDS rows
-    if (j == 0)
-      pole = s * hc_0 / cj.fdlat2 * cj.fcos / N;
-    else
-      pole = -s * hc_m / cj.fdlat2 * cj.fcos / N;
-  } else {
-    __syncthreads();
-  }
-  if (tid >= W) return;
-  const int st_u = has_u ? cj.stage[CDU] : -1;
-  const int st_v = has_v ? cj.stage[CDV] : -1;
-  const int st_g = cj.stage[CDGD];
-  double *sx = mb.sx();
-  const double kd = a.kdiff;
-  double *Xgd = mb.st(SX, FGD), *Xu = mb.st(SX, Fu), *Xv = mb.st(SX, Fv);
-  const int i = i0 + tid, li = tid + 2;
-  const size_t o = (size_t)j * P;
-  const double g0 = sgd[1][li];
-  double gdd;
-  if (pole_row) {
-    gdd = pole;
-  } else {
-    gdd = (sgd[1][li + 1] - 2 * g0 + sgd[1][li - 1]) / cj.fdlon2 +
-          ((sgd[2][li] - g0) * hc_0 - (g0 - sgd[0][li]) * hc_m) / cj.fdlat2 * cj.fcos;
-  }
-  double ud = 0.0;
-  if (has_u) {
-    const double u0 = su[1][li];
-    ud = (su[1][li + 1] - 2 * u0 + su[1][li - 1]) / cj.fdlon2 +
-         ((su[2][li] - u0) * hc_0 - (u0 - su[0][li]) * hc_m) / cj.fdlat2 * cj.fcos;
-  }
-  double vd = 0.0;
-  if (has_v) {
-    const double v0 = sv[1][li];
-    vd = (sv[1][li + 1] - 2 * v0 + sv[1][li - 1]) / cj.hdlon2;
-    if (j >= 1 && j <= M - 3)
-      vd = vd + ((sv[2][li] - v0) * fc_p - (v0 - sv[0][li]) * cj.fcos) / cj.hdlat2 * cj.hcos;
-    else if (j == 0)
-      vd = vd + (sv[2][li] - v0) * fc_p / cj.hdlat2 * cj.hcos;
-    else
-      vd = vd - (v0 - sv[0][li]) * cj.fcos / cj.hdlat2 * cj.hcos;
-  }
-  if (!a.lap_final) {
-    // intermediate application: the result is the next operand (pole rows
-    // of u stay zero; diffusion_mod.F90:172-179)
-    mb.st(a.s_lapout, FGD)[o + i] = gdd;
-    mb.st(a.s_lapout, Fu)[o + i] = ud;
-    if (has_v) mb.st(a.s_lapout, Fv)[o + i] = vd;
-    return;
-  }
-  const double sg = prev ? x_g : g0;
-  const double su0 = prev ? x_u : su[1][li];
-  const double sv0 = prev ? x_v : sv[1][li];
-  if (st_g >= 0)
-    sx[(size_t)st_g * P + i] = gdd;
-  else
-    Xgd[o + i] = sg + kd * gdd;
-  if (st_u >= 0)
-    sx[(size_t)st_u * P + i] = ud;
-  else
-    Xu[o + i] = has_u ? su0 + kd * ud : 0.0;
-  if (has_v) {
-    if (st_v >= 0)
-      sx[(size_t)st_v * P + i] = vd;
-    else
-      Xv[o + i] = sv0 + kd * vd;
-  }
+  dif_tile<false>(a, t, smem, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -596,10 +446,11 @@ hipError_t launch_update(const Args &a, int prec, hipStream_t s) {
 
 hipError_t launch_diffuse(const Args &a, int prec, hipStream_t s) {
   dim3 g(row_grid(a.ntiles)), b(kBlock);
+  const size_t lds = dif_stencil_bytes();
   if (prec == 0)
-    hipLaunchKernelGGL(k_diffuse<double>, g, b, 0, s, a);
+    hipLaunchKernelGGL(k_diffuse<double>, g, b, lds, s, a);
   else
-    hipLaunchKernelGGL(k_diffuse<float>, g, b, 0, s, a);
+    hipLaunchKernelGGL(k_diffuse<float>, g, b, lds, s, a);
   return hipGetLastError();
 }
 

@@ -14,7 +14,8 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // k_filter: twiddle table (N complex) + two FFT buffers of L complex points.
 size_t filter_lds_bytes(int N, int L);
 
-// k_tend: one segment's stencil rows.
+// k_tend: one segment's stencil rows.  phase_lds_bytes: the larger of the
+// fused phase and fused diffusion launches.
 size_t tend_lds_bytes(const Args &a, size_t cts);
 
 // prec: 0 = FP64 plain, 1 = mixed (FP32 compute type)
@@ -39,6 +40,9 @@ hipError_t configure_filter(int max_lds);
 // one tendency phase with its filter rows in one launch (gd_phase.hip);
 // N <= 4 * kBlock
 hipError_t launch_phase(const Args &a, int prec, int pass, bool in_full, hipStream_t s);
+// the last diffusion application with its filter rows in one launch
+// (gd_phase.hip); N <= 4 * kBlock
+hipError_t launch_dphase(const Args &a, int prec, hipStream_t s);
 size_t phase_lds_bytes(int N, int L, int prec);
 hipError_t configure_phase(int max_lds);
 

@@ -20,6 +20,7 @@
 #include "gd_fft.hpp"
 #include "gd_launch.hpp"
 #include "gd_tend.hpp"
+#include "gd_diffuse.hpp"
 
 namespace gd {
 
@@ -137,8 +138,103 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 
   tend_tile<CT, PASS, IN_FULL, true>(a, t, smem, red);
 }
 
+// ---------------------------------------------------------------------------
+// k_dphase: the last application of ordinary_diffusion with its filter rows
+// (diffusion_mod.F90:74-217 and the filter of dycore_mod.F90:658-666) in one
+// launch, in place of k_diffuse followed by k_filter; same bits.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline size_t dphase_region0(int L) {
+  const size_t st = dif_stencil_bytes(), fb = (size_t)L * 16;
+  return align16(st > fb ? st : fb);
+}
+
+__host__ __device__ inline size_t dphase_lds(int N, int L) {
+  return dphase_region0(L) + (size_t)L * 16 + (size_t)tw_entries(N) * 16;
+}
+
+template <int CM>
+__device__ __forceinline__ void dnear_row(const Args &a, const Member &mb, const RowCoef &cj, int j, double *rowbuf,
+                                          unsigned char *smem, double *red) {
+  const int N = a.N, tid = threadIdx.x;
+  for (int g = 0; g < a.nseg; ++g) {
+    if (g) __syncthreads();  // the previous segment's stencil rows have been read
+    const int i0 = g * kSeg, W = min(N - i0, kSeg);
+    const DiffPoint d = dif_segment<CM>(a, mb, cj, j, i0, W, smem, red);
+    if (tid < W) row_put(a, rowbuf, i0 + tid, CM == TC_DU ? d.ud : (CM == TC_DV ? d.vd : d.gdd));
+  }
+}
+
+template <typename CT, int KP>
+__device__ __forceinline__ void dnear_tile(const Args &a, const FiltRow fr, unsigned char *smem, double *red) {
+  const int N = a.N, L = a.fft_L, tid = threadIdx.x;
+  const Member mb = member_at(a, fr.member);
+  const int j = fr.row;
+  const RowCoef cj = a.coef[j];
+  double2 *b1 = (double2 *)smem;
+  double2 *b0 = (double2 *)(smem + dphase_region0(L));
+  double2 *tw = b0 + L;
+  constexpr int TK = KP / 2 + 1;
+  const int nt = tw_entries(N);
+  double2 twr[TK];
+#pragma unroll
+  for (int k = 0; k < TK; ++k) {
+    const int e = tid + k * kBlock;
+    twr[k] = e < nt ? make_double2(a.twid[2 * e], a.twid[2 * e + 1]) : make_double2(0.0, 0.0);
+  }
+  double *rowbuf = (double *)b0;
+  if (fr.comp == CDU) {
+    dnear_row<TC_DU>(a, mb, cj, j, rowbuf, smem, red);
+  } else if (fr.comp == CDV) {
+    dnear_row<TC_DV>(a, mb, cj, j, rowbuf, smem, red);
+  } else if (j == 0 || j == a.M - 1) {
+    // a pole row's gdd is the same zonal sum at every longitude
+    const double pole = dif_pole(a, mb, j, red);
+    for (int l = tid; l < N; l += kBlock) row_put(a, rowbuf, l, pole);
+  } else {
+    dnear_row<TC_DGD>(a, mb, cj, j, rowbuf, smem, red);
+  }
+#pragma unroll
+  for (int k = 0; k < TK; ++k) {
+    const int e = tid + k * kBlock;
+    if (e < nt) tw[e] = twr[k];
+  }
+  __syncthreads();  // row and twiddles published; the stencil rows are free for b1
+  filter_row<CT, KP, true>(a, mb, fr, tw, b0, b1, red);
+}
+
+template <typename CT, int KP>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) k_dphase(const Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[4 * 6];
+  const int b = blockIdx.x;
+  if (b < a.nnear_pad) {
+    const int t = (b % kXcd) * (a.nnear_pad / kXcd) + b / kXcd;
+    if (t < a.nnear) dnear_tile<CT, KP>(a, a.frows[t], smem, red);
+    return;
+  }
+  const int bb = b - a.nnear_pad, per = (gridDim.x - a.nnear_pad) / kXcd;
+  const int t = (bb % kXcd) * per + bb / kXcd;
+  if (t >= a.ntiles) return;
+  dif_tile<true>(a, t, smem, red);
+}
+
+hipError_t launch_dphase(const Args &a, int prec, hipStream_t s) {
+  if (a.N > 4 * kBlock) return hipErrorInvalidValue;
+  const int rows = ((a.ntiles + kXcd - 1) / kXcd) * kXcd;
+  dim3 g(a.nnear_pad + rows), b(kBlock);
+  const size_t lds = dphase_lds(a.N, a.fft_L);
+  if (prec == 0)
+    hipLaunchKernelGGL((k_dphase<double, 4>), g, b, lds, s, a);
+  else
+    hipLaunchKernelGGL((k_dphase<float, 4>), g, b, lds, s, a);
+  return hipGetLastError();
+}
+
 size_t phase_lds_bytes(int N, int L, int prec) {
-  return prec == 0 ? phase_lds_t<double>(N, L) : phase_lds_t<float>(N, L);
+  const size_t p = prec == 0 ? phase_lds_t<double>(N, L) : phase_lds_t<float>(N, L);
+  const size_t d = dphase_lds(N, L);
+  return p > d ? p : d;
 }
 
 template <typename CT, int KP>
@@ -174,7 +270,11 @@ static hipError_t configure_phase_k(int max_lds) {
 hipError_t configure_phase(int max_lds) {
   hipError_t e;
   if ((e = configure_phase_k<double, 4>(max_lds)) != hipSuccess) return e;
-  return configure_phase_k<float, 4>(max_lds);
+  if ((e = configure_phase_k<float, 4>(max_lds)) != hipSuccess) return e;
+  if ((e = hipFuncSetAttribute((const void *)k_dphase<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               max_lds)) != hipSuccess)
+    return e;
+  return hipFuncSetAttribute((const void *)k_dphase<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
 }
 
 }  // namespace gd

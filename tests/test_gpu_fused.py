@@ -1,6 +1,7 @@
-"""The fused phase launch (k_phase: a tendency phase and its polar filter rows
-in one launch) against k_tend followed by k_filter (GD_FUSED=0 when the
-handle is created).  Both evaluate every value with the same expressions in
+"""The fused launches (k_phase: a tendency phase and its polar filter rows in
+one launch; k_dphase: the last diffusion application and its filter rows)
+against k_tend / k_diffuse followed by k_filter (GD_FUSED=0 when the handle
+is created).  Both evaluate every value with the same expressions in
 the same order, so they must agree bit for bit: states, per-step mass, energy
 and beta, for every split scheme, both precisions, both advection schemes,
 even and odd num_lon, and ensembles with a subset of active members.
@@ -68,6 +69,8 @@ CASES = [
     (45, 23, 900.0, 4, dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_coef=2e5)),
     (360, 181, 240.0, 3, dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=1e5)),
     (1024, 33, 30.0, 2, dict(split_scheme="csp2", subcycles=2, use_diffusion=True, diffusion_coef=1e4)),
+    (72, 37, 900.0, 4, dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_order=4,
+                            diffusion_coef=2e5 * 4e10)),
 ]
 
 
