@@ -126,6 +126,18 @@ __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, 
   const double g_i = L.g_i, g_ip = L.g_ip, h_i = L.h_i, h_ip = L.h_ip, g_n = L.g_n, h_n = L.h_n;
   const int lslot = tid < W ? tid + 2 : (tid == W ? 1 : W + 2);
   const bool own = tid < W;
+  // LDS rows the components of this pass read at i-1 / i+1 (PASS 1: the
+  // fast terms fv, upgf, fu, vpgf and the mass flux; PASS 2: the advection
+  // terms); rows nobody reads are not written, so their loads and the u, v
+  // they would derive drop out
+  constexpr bool W_G1 = PASS != 2 && (DU || DG);                            // upgf, mass flux
+  constexpr bool W_U0 = (DU && PASS != 1) || ((DV || DG) && PASS != 2);     // ual; fu, mass flux
+  constexpr bool W_U1 = DV && PASS != 2;                                    // fu
+  constexpr bool W_V0 = DU && PASS != 2;                                    // fv
+  constexpr bool W_V1 = (DU && PASS != 2) || (DV && PASS != 1);             // fv; val
+  constexpr bool W_u0 = (DU && PASS != 1) || DV;                            // ual; fu, val
+  constexpr bool W_u1 = DV;                                                 // fu, val
+  constexpr bool W_v = DU && PASS != 1;                                     // uat
   // ---- LDS rows; the own-longitude rows stay in registers ----
   double G[4];
   CT u_own[2] = {(CT)0, (CT)0}, v_own3[3] = {(CT)0, (CT)0, (CT)0};  // u rows j, j+1; v half rows j-1..j+1
@@ -133,11 +145,11 @@ __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, 
     const int li = lslot;
 #pragma unroll
     for (int r = 0; r < 4; ++r) G[r] = sqrt(lg[r]);
-    if (DU || DG) sG1[li] = G[1];
-    sU[0][li] = lU[1];
-    if (DV) sU[1][li] = lU[2];
-    if (DU) sV[0][li] = lV[0];
-    if (DU || DV) sV[1][li] = lV[1];
+    if (W_G1) sG1[li] = G[1];
+    if (W_U0) sU[0][li] = lU[1];
+    if (W_U1) sU[1][li] = lU[2];
+    if (W_V0) sV[0][li] = lV[0];
+    if (W_V1) sV[1][li] = lV[1];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int row = j + r;
@@ -148,7 +160,7 @@ __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, 
         // update_state: u = U * 2 / (G(i) + G(i+1))  (dycore_mod.F90:636-640)
         val = ((CT)lU[r + 1] * (CT)2) / ((CT)G[r + 1] + (CT)sqrt(lgn[r]));
       }
-      if ((r == 0 && (DU || DV)) || DV) su[r][li] = val;
+      if ((r == 0 && W_u0) || (r == 1 && W_u1)) su[r][li] = val;
       u_own[r] = val;
     }
 #pragma unroll
@@ -161,7 +173,7 @@ __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, 
         else  // v = V * 2 / (G(j) + G(j+1))  (dycore_mod.F90:641-645)
           val = ((CT)lV[r] * (CT)2) / ((CT)G[r] + (CT)G[r + 1]);
       }
-      if (r < 2 && DU) sv[r][li] = val;
+      if (r < 2 && W_v) sv[r][li] = val;
       v_own3[r] = val;
     }
   }
