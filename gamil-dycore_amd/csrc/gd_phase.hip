@@ -54,6 +54,7 @@ template <typename CT, int PASS, bool IN_FULL, int CM>
 __device__ __forceinline__ void near_row(const Args &a, const Member &mb, const RowCoef &cj, int j, double *rowbuf,
                                          unsigned char *smem, double *red) {
   const int N = a.N, tid = threadIdx.x;
+  const RowNbr nb = row_nbr(a, j);
   SegLoads cur;
   seg_load<PASS, IN_FULL>(a, mb, j, 0, min(N, kSeg), cur);
   for (int g = 0; g < a.nseg; ++g) {
@@ -61,7 +62,7 @@ __device__ __forceinline__ void near_row(const Args &a, const Member &mb, const 
     SegLoads nxt;
     if (g + 1 < a.nseg) seg_load<PASS, IN_FULL>(a, mb, j, i0 + kSeg, min(N - i0 - kSeg, kSeg), nxt);
     if (g) __syncthreads();  // the previous segment's stencil rows have been read
-    const TendPoint pt = seg_eval<CT, PASS, IN_FULL, CM>(a, cj, j, i0, W, cur, false, 0.0,
+    const TendPoint pt = seg_eval<CT, PASS, IN_FULL, CM>(a, cj, nb, j, i0, W, cur, false, 0.0,
                                                          TendPoint{0, 0, 0, 0, 0, 0}, smem, red);
     if (tid < W) row_put(a, rowbuf, i0 + tid, CM == TC_DU ? pt.du : (CM == TC_DV ? pt.dv : pt.dgd));
     cur = nxt;

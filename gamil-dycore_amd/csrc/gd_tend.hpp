@@ -99,13 +99,30 @@ __device__ __forceinline__ void seg_load(const Args &a, const Member &mb, int j,
   }
 }
 
+// Coefficients of the neighbouring rows j-1 and j+1 (zero beyond the grid).
+// Loaded once per tile, before its first barrier: a barrier orders memory,
+// so loads written after it are issued after it.
+struct RowNbr {
+  double hc_m, hc_p, f_p, c_p;
+};
+
+__device__ __forceinline__ RowNbr row_nbr(const Args &a, int j) {
+  const int M = a.M;
+  RowNbr nb;
+  nb.hc_m = j >= 1 ? a.coef[j - 1].hcos : 0.0;
+  nb.hc_p = j + 1 <= M - 2 ? a.coef[j + 1].hcos : 0.0;
+  nb.f_p = j + 1 <= M - 1 ? a.coef[j + 1].f : 0.0;
+  nb.c_p = j + 1 <= M - 1 ? a.coef[j + 1].c : 0.0;
+  return nb;
+}
+
 // The rest of a segment once its loads are issued: the LDS rows, one
 // barrier (or the pole ring sum pacc's block reduction when pole_row) and
 // the components CM of the own point, into out.  The LDS rows only other
 // components read are not written, so their loads drop out.
 template <typename CT, int PASS, bool IN_FULL, int CM>
-__device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, int j, int i0, int W,
-                                              const SegLoads &L, bool pole_row, double pacc, TendPoint out,
+__device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, const RowNbr &nb, int j, int i0,
+                                              int W, const SegLoads &L, bool pole_row, double pacc, TendPoint out,
                                               unsigned char *smem, double *red) {
   const int N = a.N, M = a.M;
   double *sG1 = (double *)smem;                          // G row j
@@ -187,11 +204,7 @@ __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, 
   }
   if (!own) return out;
 
-  const double hc_m = j >= 1 ? a.coef[j - 1].hcos : 0.0;
-  const double hc_0 = has_v ? cj.hcos : 0.0;
-  const double hc_p = j + 1 <= M - 2 ? a.coef[j + 1].hcos : 0.0;
-  const double f_p = j + 1 <= M - 1 ? a.coef[j + 1].f : 0.0;
-  const double c_p = j + 1 <= M - 1 ? a.coef[j + 1].c : 0.0;
+  const double hc_m = nb.hc_m, hc_0 = has_v ? cj.hcos : 0.0, hc_p = nb.hc_p, f_p = nb.f_p, c_p = nb.c_p;
   const CT bl = (CT)a.beta_lon, bt = (CT)a.beta_lat;
   const bool upwind = a.upwind != 0;
   const int li = tid + 2;
@@ -318,6 +331,7 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
     for (int q = tid; q < N; q += kBlock)
       pacc += (sqrt(gd[(size_t)j * P + q]) + sqrt(gd[(size_t)jn * P + q])) * Vg[(size_t)hr * P + q];
   }
+  const RowNbr nb = row_nbr(a, j);
   SegLoads L;
   seg_load<PASS, IN_FULL>(a, mb, j, i0, W, L);
   // the epilogue inputs of the own point
@@ -328,7 +342,7 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
     if (!stg_v && j <= M - 2) out.pre_v = phase_pre<CT>(a, mb, CDV, o);
     if (!stg_g) out.pre_g = phase_pre<CT>(a, mb, CDGD, o);
   }
-  return seg_eval<CT, PASS, IN_FULL, CM>(a, cj, j, i0, W, L, pole_row, pacc, out, smem, red);
+  return seg_eval<CT, PASS, IN_FULL, CM>(a, cj, nb, j, i0, W, L, pole_row, pacc, out, smem, red);
 }
 
 // Row tile t = (member, row, segment of kSeg longitudes) of a phase launch.
