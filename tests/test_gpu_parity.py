@@ -70,12 +70,14 @@ def test_space_operators_unfiltered_rows_bitwise(gd, oracle_mod):
 # (72 -> 36), radix 8/8 (128 -> 64), radix 3/5 with an odd half length
 # (90 -> 45), odd N (75, full complex transform), radix 7 (70 -> 35), a
 # prime half length handled by the direct O(R^2) pass (94 -> 47), radix
-# 8 x 2 x 3 x 3 (288 -> 144), the 720- and 1440-longitude grids, and the
-# 16-points-per-thread path up to the largest supported row (2560, 4096).
+# 8 x 2 x 3 x 3 (288 -> 144), the plans fixed at compile time (180, 360,
+# 720 longitudes: half lengths 90, 180, 360), the 1440-longitude grid, and
+# the 16-points-per-thread path up to the largest supported row (2560, 4096).
 @pytest.mark.parametrize("N,M,cut", [(72, 37, [6, 5, 4, 3, 2, 1]), (128, 65, [30, 20, 10, 60, 61]),
                                      (90, 46, [44, 1, 43]), (75, 38, [36, 20, 1, 7]),
                                      (70, 36, [34, 2, 17]), (94, 48, [46, 3, 12]),
                                      (288, 145, [143, 100, 71, 9, 1]),
+                                     (180, 91, [89, 60, 20, 3, 1]), (360, 181, [179, 120, 45, 7, 1]),
                                      (720, 361, [359, 300, 200, 120, 60, 30, 15, 8, 4, 2, 1]),
                                      (1440, 721, [719, 500, 360, 101, 33, 2]),
                                      (2560, 41, [1279, 700, 31, 2]), (4096, 33, [2047, 1000, 64, 1])])
