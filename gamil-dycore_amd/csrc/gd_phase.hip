@@ -11,6 +11,11 @@
 // over the rows that have a component outside the filter.  Every value is
 // formed by the same expressions in the same order as with k_tend +
 // k_filter, so both paths give the same bits.
+//
+// k_dphase does the same for the last application of ordinary_diffusion:
+// its filter tiles evaluate their component's Laplacian over the row
+// (dif_segment, gd_diffuse.hpp) and filter it in phase D; its row tiles are
+// those of k_diffuse.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
