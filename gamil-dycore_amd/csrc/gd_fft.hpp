@@ -135,7 +135,19 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], Tw tw, int rstep, boo
 // One Stockham pass of radix R over a row of L points: butterfly j reads
 // points j + q L/R, twiddles them by W_{Ns R}^{q (j mod Ns)} and writes its
 // outputs to ((j - j mod Ns) R + j mod Ns) + q Ns.
-template <int R>
+// LDS slot of point e in a buffer written with the bank swizzle: the low
+// three index bits XOR the next three (a permutation within each group of
+// eight points, its own inverse).
+template <bool SW>
+__device__ __forceinline__ unsigned swz(unsigned e) {
+  return SW ? e ^ ((e >> 3) & 7u) : e;
+}
+
+// SWI / SWO: the source / destination buffer uses the bank swizzle.  The
+// first pass of a radix-8 plan writes each butterfly's eight outputs to
+// consecutive points, a 128-byte stride across the lanes (eight-way bank
+// conflicts); swizzled, the lanes cover all banks.
+template <int R, bool SWI = false, bool SWO = false>
 __device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L, int Ns, Tw tw,
                                          int tstride, bool inv) {
   const unsigned Lr = L / R;
@@ -145,7 +157,7 @@ __device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L
     const unsigned k = j % (unsigned)Ns;
     double2 v[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = src[j + q * Lr];
+    for (int q = 0; q < R; ++q) v[q] = src[swz<SWI>(j + q * Lr)];
     if (Ns > 1) {
 #pragma unroll
       for (int q = 1; q < R; ++q) v[q] = cmul(v[q], twf(tw, q * (int)k * tws, inv));
@@ -153,7 +165,7 @@ __device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L
     dft_small<R>(v, tw, rstep, inv);
     const unsigned o = (j - k) * R + k;
 #pragma unroll
-    for (int q = 0; q < R; ++q) dst[o + q * Ns] = v[q];
+    for (int q = 0; q < R; ++q) dst[swz<SWO>(o + q * Ns)] = v[q];
   }
 }
 
@@ -206,14 +218,23 @@ __device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const 
 // every Ns are constants, so the index arithmetic of each pass (j mod Ns,
 // the twiddle strides) folds to shifts and constant multiplies.  The
 // arithmetic on the data is that of fft_run, so both give the same bits.
-template <int L, int TS, int NS, int R, int... REST>
-__device__ __forceinline__ int fft_fixed(double2 *b0, double2 *b1, int cur, Tw tw, bool inv) {
-  fft_pass<R>(cur ? b1 : b0, cur ? b0 : b1, L, NS, tw, TS, inv);
+// The first radix-8 pass of a length divisible by 8 writes swizzled and the
+// second pass reads swizzled (only between those two passes does a buffer
+// hold the swizzled layout).
+template <int L, int TS, int NS, bool SWI, int R, int... REST>
+__device__ __forceinline__ int fft_fixed_p(double2 *b0, double2 *b1, int cur, Tw tw, bool inv) {
+  constexpr bool SWO = NS == 1 && R == 8 && L % 8 == 0 && sizeof...(REST) > 0;
+  fft_pass<R, SWI, SWO>(cur ? b1 : b0, cur ? b0 : b1, L, NS, tw, TS, inv);
   __syncthreads();
   if constexpr (sizeof...(REST) > 0)
-    return fft_fixed<L, TS, NS * R, REST...>(b0, b1, cur ^ 1, tw, inv);
+    return fft_fixed_p<L, TS, NS * R, SWO, REST...>(b0, b1, cur ^ 1, tw, inv);
   else
     return cur ^ 1;
+}
+
+template <int L, int TS, int NS, int... RS>
+__device__ __forceinline__ int fft_fixed(double2 *b0, double2 *b1, int cur, Tw tw, bool inv) {
+  return fft_fixed_p<L, TS, NS, false, RS...>(b0, b1, cur, tw, inv);
 }
 
 // The plans of the even row lengths 720, 360 and 180 (fft_plan of their
