@@ -624,7 +624,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->dcoef = (RowCoef *)dalloc(h, sizeof(RowCoef) * h->M, e);
   h->dslots = (StageSlot *)dalloc(h, sizeof(StageSlot) * std::max<size_t>(nslots, 1), e);
   h->dfrows = (FiltRow *)dalloc(h, sizeof(FiltRow) * std::max<size_t>(h->frows.size(), 1), e);
-  h->dtwid = (double *)dalloc(h, (size_t)tw_entries(h->N) * 2 * 8, e);
+  h->dtwid = (double *)dalloc(h, (size_t)tw_padded(h->N) * 2 * 8, e);
   h->dstep = (int32_t *)dalloc(h, sizeof(int32_t) * h->nmem, e);
   if (!h->ddiag || !h->dcoef || !h->dslots || !h->dfrows || !h->dtwid ||
       !h->dstep) {
@@ -634,7 +634,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   // twiddle table of the filter FFT: cos and sin of 2 pi k/N for k =
   // 0..N/2, evaluated in extended precision and rounded once to FP64 (the
   // kernels take W_N^k for k > N/2 as the conjugate of W_N^(N-k))
-  std::vector<double> tw((size_t)tw_entries(h->N) * 2);
+  std::vector<double> tw((size_t)tw_padded(h->N) * 2, 0.0);
   for (int k = 0; k < tw_entries(h->N); ++k) {
     const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)h->N;
     tw[2 * k] = (double)cosl(ang);

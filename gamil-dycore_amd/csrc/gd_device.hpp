@@ -33,6 +33,9 @@ constexpr int kMaxFarRng = 4;    // row ranges of a fused phase launch's row til
 
 // Entries k = 0..N/2 of the filter's twiddle table (the others are conjugates).
 __host__ __device__ inline int tw_entries(int N) { return N / 2 + 1; }
+// The table as stored (device and LDS): padded with zeros to whole waves of
+// 64 entries, so a wave copies it straight to LDS with no lane out of range.
+__host__ __device__ inline int tw_padded(int N) { return (tw_entries(N) + 63) / 64 * 64; }
 
 // Per full row j (0-based).  Half-row entries at index j describe half row j
 // (0 for j = M-1).

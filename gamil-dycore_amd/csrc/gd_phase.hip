@@ -39,7 +39,19 @@ __host__ __device__ inline size_t phase_region0(int L) {
 
 template <typename CT>
 __host__ __device__ inline size_t phase_lds_t(int N, int L) {
-  return phase_region0<CT>(L) + (size_t)L * 16 + (size_t)tw_entries(N) * 16;
+  return phase_region0<CT>(L) + (size_t)L * 16 + (size_t)tw_padded(N) * 16;
+}
+
+// The twiddle table straight from HBM to LDS (global_load_lds: no registers
+// hold it in flight).  Each wave copies whole 64-entry blocks; a lane moves
+// one 16-byte entry, and the wave's block lands contiguously from the LDS
+// address it names.  The copy is complete after s_waitcnt vmcnt(0).
+__device__ __forceinline__ void tw_to_lds(const Args &a, double2 *tw) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, np = tw_padded(a.N);
+  for (int base = w * 64; base < np; base += kBlock) {
+    __builtin_amdgcn_global_load_lds((const void *)(a.twid + 2 * (base + lane)),
+                                     (__attribute__((address_space(3))) void *)(tw + base), 16, 0, 0);
+  }
 }
 
 // Row value x of longitude l into the FFT layout of b0.
@@ -83,16 +95,7 @@ __device__ __forceinline__ void near_tile(const Args &a, const FiltRow fr, unsig
   double2 *b1 = (double2 *)smem;
   double2 *b0 = (double2 *)(smem + phase_region0<CT>(L));
   double2 *tw = b0 + L;
-  // the twiddle loads are issued first and reach LDS after the tendency
-  // loop, so their latency overlaps the first segment's
-  constexpr int TK = KP / 2 + 1;  // tw_entries(KP * kBlock) <= TK * kBlock
-  const int nt = tw_entries(N);
-  double2 twr[TK];
-#pragma unroll
-  for (int k = 0; k < TK; ++k) {
-    const int e = tid + k * kBlock;
-    twr[k] = e < nt ? make_double2(a.twid[2 * e], a.twid[2 * e + 1]) : make_double2(0.0, 0.0);
-  }
+  tw_to_lds(a, tw);
   double *rowbuf = (double *)b0;
   if (fr.comp == CDU) {
     near_row<CT, PASS, IN_FULL, TC_DU>(a, mb, cj, j, rowbuf, smem, red);
@@ -109,11 +112,7 @@ __device__ __forceinline__ void near_tile(const Args &a, const FiltRow fr, unsig
   } else {
     near_row<CT, PASS, IN_FULL, TC_DGD>(a, mb, cj, j, rowbuf, smem, red);
   }
-#pragma unroll
-  for (int k = 0; k < TK; ++k) {
-    const int e = tid + k * kBlock;
-    if (e < nt) tw[e] = twr[k];
-  }
+  __builtin_amdgcn_s_waitcnt(0);  // the twiddle copy has landed
   __syncthreads();  // row and twiddles published; the stencil rows are free for b1
   filter_row<CT, KP, true>(a, mb, fr, tw, b0, b1, red);
 }
@@ -156,7 +155,7 @@ __host__ __device__ inline size_t dphase_region0(int L) {
 }
 
 __host__ __device__ inline size_t dphase_lds(int N, int L) {
-  return dphase_region0(L) + (size_t)L * 16 + (size_t)tw_entries(N) * 16;
+  return dphase_region0(L) + (size_t)L * 16 + (size_t)tw_padded(N) * 16;
 }
 
 template <int CM>
@@ -180,14 +179,7 @@ __device__ __forceinline__ void dnear_tile(const Args &a, const FiltRow fr, unsi
   double2 *b1 = (double2 *)smem;
   double2 *b0 = (double2 *)(smem + dphase_region0(L));
   double2 *tw = b0 + L;
-  constexpr int TK = KP / 2 + 1;
-  const int nt = tw_entries(N);
-  double2 twr[TK];
-#pragma unroll
-  for (int k = 0; k < TK; ++k) {
-    const int e = tid + k * kBlock;
-    twr[k] = e < nt ? make_double2(a.twid[2 * e], a.twid[2 * e + 1]) : make_double2(0.0, 0.0);
-  }
+  tw_to_lds(a, tw);
   double *rowbuf = (double *)b0;
   if (fr.comp == CDU) {
     dnear_row<TC_DU>(a, mb, cj, j, rowbuf, smem, red);
@@ -200,11 +192,7 @@ __device__ __forceinline__ void dnear_tile(const Args &a, const FiltRow fr, unsi
   } else {
     dnear_row<TC_DGD>(a, mb, cj, j, rowbuf, smem, red);
   }
-#pragma unroll
-  for (int k = 0; k < TK; ++k) {
-    const int e = tid + k * kBlock;
-    if (e < nt) tw[e] = twr[k];
-  }
+  __builtin_amdgcn_s_waitcnt(0);  // the twiddle copy has landed
   __syncthreads();  // row and twiddles published; the stencil rows are free for b1
   filter_row<CT, KP, true>(a, mb, fr, tw, b0, b1, red);
 }
