@@ -1,6 +1,8 @@
-// gd_fft.hpp -- the polar zonal filter of one staged tendency row
+// gd_fft.hpp -- the polar zonal filter of one tendency row
 // (filter_mod.F90:105-172) fused with the rescale of space_operators
-// (dycore_mod.F90:210-217) and the phase epilogue; used by k_filter.
+// (dycore_mod.F90:210-217) and the phase epilogue; used by k_filter (staged
+// rows) and by the filter tiles of k_phase and k_dphase (rows evaluated in
+// LDS by the tile itself).
 //
 // FFTPACK5's rfft1f -> mask -> rfft1b of one row is the orthogonal projection
 // onto the Fourier modes the mask keeps: wavenumbers 0..c and the cosine part
