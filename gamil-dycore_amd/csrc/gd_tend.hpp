@@ -88,14 +88,18 @@ __device__ __forceinline__ void seg_load(const Args &a, const Member &mb, int j,
     L.lV[r] = on && row >= 0 && row <= M - 2 ? Vg[(size_t)row * P + p] : 0.0;
     L.lfv[r] = IN_FULL && on && row >= 0 && row <= M - 2 ? mb.st(a.s_in, Fv)[(size_t)row * P + p] : 0.0;
   }
+  // the own point's geopotential: gd at (j, i), (j, i+1), (j+1, i) is the
+  // column load already made (lg[1], lgn[0], lg[2]); only ghs is loaded
   const int i = i0 + tid;
   const int ip = i + 1 >= N ? i + 1 - N : i + 1;
   const size_t o = (size_t)j * P;
-  L.g_i = L.g_ip = L.h_i = L.h_ip = L.g_n = L.h_n = 0.0;
+  L.g_i = L.lg[1];
+  L.g_ip = L.lgn[0];
+  L.g_n = L.lg[2];
+  L.h_i = L.h_ip = L.h_n = 0.0;
   if (tid < W && PASS != 2) {
-    L.g_i = gd[o + i]; L.g_ip = gd[o + ip];
     L.h_i = ghs[o + i]; L.h_ip = ghs[o + ip];
-    if (j <= M - 2) { L.g_n = gd[o + P + i]; L.h_n = ghs[o + P + i]; }
+    if (j <= M - 2) L.h_n = ghs[o + P + i];
   }
 }
 
