@@ -303,12 +303,15 @@ __device__ __forceinline__ void cplx_mid(const double2 *s, double2 *d, int L, in
 // fills the twiddle table and the row buffer b0 itself.  XIN = true: the
 // caller has written the row to b0 in the FFT layout (even N: N reals;
 // odd N: N complex points with zero imaginary parts) and the twiddle table
-// to tw, followed by a barrier.  KP: longitudes per thread (N <= KP *
+// to tw, followed by a barrier.  zero (XIN only, rescaled phases only): the
+// row is known to be zero and b0 is not read; with s1 = 0 the row stays
+// unfiltered, so the epilogue gets T = +0 with no transform, as the
+// transform path would give it.  KP: longitudes per thread (N <= KP *
 // kBlock).  tw holds tw_entries(N) entries, b0 and b1 fft_L complex points
 // each; red holds 8 doubles.  Ends with a barrier.
 template <typename CT, int KP, bool XIN>
 __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, const FiltRow &fr, double2 *tw,
-                                           double2 *b0, double2 *b1, double *red) {
+                                           double2 *b0, double2 *b1, double *red, bool zero = false) {
   const int N = a.N, P = a.pitch, L = a.fft_L, tid = threadIdx.x;
   const bool rescale = a.phase <= PH_T;
   const double sc = 1.0 / (double)L;
@@ -333,8 +336,10 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
     if (l < N) {
       if (!XIN) xv = xp[l];
       if (rescale) {
-        wv = wp[l];
-        if (gdrow) gv = wq[l];
+        if (!zero) {
+          wv = wp[l];
+          if (gdrow) gv = wq[l];
+        }
         pv = phase_pre<CT>(a, mb, comp, o + l);
       }
     }
@@ -363,36 +368,40 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       const int l = tid + k * kBlock;
-      x[k] = l < N ? rowbuf[(size_t)l * ystep] : 0.0;
+      x[k] = l < N && !zero ? rowbuf[(size_t)l * ystep] : 0.0;
     }
   }
-
-  const Tw twt{tw, N};
-  int cur = fft_any(b0, b1, 0, a, twt, false);
-  if (gdrow) {
-#pragma unroll
-    for (int k = 0; k < KP; ++k) w[k] = w[k] + wg[k];
-  }
-  if (a.fft_half)
-    real_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, twt, sc);
-  else
-    cplx_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, sc);
-  __syncthreads();
-  cur = fft_any(b0, b1, cur ^ 1, a, twt, true);
-  const double *y = (const double *)(cur ? b1 : b0);
 
   // s1 = sum(tend * weight) before and s2 = sum(filtered tend * weight)
   // after filtering (dycore_mod.F90:211, 214), reduced together
   double yv[KP];
   double s12[2] = {0.0, 0.0};
+  if (!zero) {
+    const Tw twt{tw, N};
+    int cur = fft_any(b0, b1, 0, a, twt, false);
+    if (gdrow) {
 #pragma unroll
-  for (int k = 0; k < KP; ++k) {
-    const int l = tid + k * kBlock;
-    yv[k] = l < N ? y[(size_t)l * ystep] : 0.0;
-    s12[0] += x[k] * w[k];
-    s12[1] += yv[k] * w[k];
+      for (int k = 0; k < KP; ++k) w[k] = w[k] + wg[k];
+    }
+    if (a.fft_half)
+      real_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, twt, sc);
+    else
+      cplx_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, sc);
+    __syncthreads();
+    cur = fft_any(b0, b1, cur ^ 1, a, twt, true);
+    const double *y = (const double *)(cur ? b1 : b0);
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int l = tid + k * kBlock;
+      yv[k] = l < N ? y[(size_t)l * ystep] : 0.0;
+      s12[0] += x[k] * w[k];
+      s12[1] += yv[k] * w[k];
+    }
+    if (rescale) block_sum_n<2>(s12, red);
+  } else {
+#pragma unroll
+    for (int k = 0; k < KP; ++k) yv[k] = 0.0;
   }
-  if (rescale) block_sum_n<2>(s12, red);
   const double s1 = s12[0], s2 = s12[1];
   // rows with |s1| <= filter_inner_product_threshold stay unfiltered
   const bool filt = !rescale || fabs(s1) > 1.0e-16;

@@ -107,14 +107,16 @@ __device__ __forceinline__ void near_tile(const Args &a, const FiltRow fr, unsig
     const double pole = pole_flux(a, mb, j, red);
     for (int l = tid; l < N; l += kBlock) row_put(a, rowbuf, l, -pole);
   } else if (PASS == 2) {
-    // the slow pass has no mass continuity term
-    for (int l = tid; l < N; l += kBlock) row_put(a, rowbuf, l, 0.0);
+    // the slow pass has no mass continuity term: a zero row, which a
+    // rescaled phase leaves unfiltered without transforming it
+    if (a.phase > PH_T)
+      for (int l = tid; l < N; l += kBlock) row_put(a, rowbuf, l, 0.0);
   } else {
     near_row<CT, PASS, IN_FULL, TC_DGD>(a, mb, cj, j, rowbuf, smem, red);
   }
   __builtin_amdgcn_s_waitcnt(0);  // the twiddle copy has landed
   __syncthreads();  // row and twiddles published; the stencil rows are free for b1
-  filter_row<CT, KP, true>(a, mb, fr, tw, b0, b1, red);
+  filter_row<CT, KP, true>(a, mb, fr, tw, b0, b1, red, PASS == 2 && fr.comp == CDGD && a.phase <= PH_T);
 }
 
 // Five waves per SIMD (at most 96 VGPRs): with the LDS of an FP64 tile that

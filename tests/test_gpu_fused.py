@@ -4,7 +4,9 @@ against k_tend / k_diffuse followed by k_filter (GD_FUSED=0 when the handle
 is created).  Both evaluate every value with the same expressions in
 the same order, so they must agree bit for bit: states, per-step mass, energy
 and beta, for every split scheme, both precisions, both advection schemes,
-even and odd num_lon, and ensembles with a subset of active members.
+even and odd num_lon, and ensembles with a subset of active members.  The
+slow pass's dgd rows are zero: k_phase gives them T = +0 without a transform,
+k_filter transforms them, so the csp2 cases also check that shortcut.
 """
 import os
 from contextlib import contextmanager
