@@ -70,6 +70,10 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   size_t off[U];
   bool on[U], hv[U];
   double g0[U], u0[U], v0[U], tg[U], tu[U], tv[U];
+  // Every lane issues every load (points off the band read offset 0, inside
+  // every field), so the number of loads in flight is the same on every
+  // path and the waits below can let the later ones stay in flight.
+  const double *tu_p = isp ? rcu : tcu, *tv_p = isp ? rcv : tcv;
   auto load = [&](int base) {
 #pragma unroll
     for (int k = 0; k < U; ++k) {
@@ -78,40 +82,55 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
       const int row = r0 + (on[k] ? e / N : 0), i = on[k] ? e - (e / N) * N : 0;
       hv[k] = on[k] && row <= M - 2;
       off[k] = (size_t)row * P + i;
-      g0[k] = on[k] ? s0g[off[k]] : 0.0;
-      u0[k] = on[k] ? s0u[off[k]] : 0.0;
-      tg[k] = on[k] ? tcg[off[k]] : 0.0;
-      tu[k] = on[k] ? (isp ? rcu[off[k]] : tcu[off[k]]) : 0.0;
-      v0[k] = hv[k] ? s0v[off[k]] : 0.0;
-      tv[k] = hv[k] ? (isp ? rcv[off[k]] : tcv[off[k]]) : 0.0;
+      const size_t ou_ = on[k] ? off[k] : 0, ov_ = hv[k] ? off[k] : 0;
+      const double xg = s0g[ou_], xu = s0u[ou_], xtg = tcg[ou_], xtu = tu_p[ou_];
+      const double xv = s0v[ov_], xtv = tv_p[ov_];
+      g0[k] = on[k] ? xg : 0.0;
+      u0[k] = on[k] ? xu : 0.0;
+      tg[k] = on[k] ? xtg : 0.0;
+      tu[k] = on[k] ? xtu : 0.0;
+      v0[k] = hv[k] ? xv : 0.0;
+      tv[k] = hv[k] ? xtv : 0.0;
     }
   };
-  // the first batch of the band does not depend on beta: issue it first
-  load(0);
   // beta from the phase-C partials: inner_product(tend(old), tend(new)) and
   // (tend(new), tend(new)) (types_mod.F90:347-371) summed du, then dv, then dgd
   const int per_comp = M * nseg;
   double q[6] = {0, 0, 0, 0, 0, 0};
-  for (int k0 = 0; k0 < per_comp; k0 += 8 * kBlock) {
-    double x[8][6];
+  auto pload = [&](int k0, double(&x)[8][6]) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int k = k0 + u * kBlock + tid;
-      const int row = k / nseg, sg = k - (k / nseg) * nseg;
+      const bool in = k < per_comp;
+      const int kk = in ? k : 0;  // unconditional loads, as in load()
+      const int row = kk / nseg, sg = kk - (kk / nseg) * nseg;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        x[u][2 * c] = x[u][2 * c + 1] = 0.0;
-        if (k < per_comp) {
-          const double *pp = mb.part() + ((size_t)(row * 3 + c) * nseg + sg) * 2;
-          x[u][2 * c] = pp[0];
-          x[u][2 * c + 1] = pp[1];
-        }
+        const double2 v = *(const double2 *)(mb.part() + ((size_t)(row * 3 + c) * nseg + sg) * 2);
+        x[u][2 * c] = in ? v.x : 0.0;
+        x[u][2 * c + 1] = in ? v.y : 0.0;
       }
     }
+  };
+  auto pacc = [&](const double(&x)[8][6]) {
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
       for (int c = 0; c < 6; ++c) q[c] += x[u][c];
+  };
+  {
+    // the first partials, then the band's first batch, which does not depend
+    // on beta: memory returns in issue order, so the reduction waits only
+    // for the partials and overlaps the batch's loads
+    double x[8][6];
+    pload(0, x);
+    load(0);
+    pacc(x);
+  }
+  for (int k0 = 8 * kBlock; k0 < per_comp; k0 += 8 * kBlock) {
+    double x[8][6];
+    pload(k0, x);
+    pacc(x);
   }
   block_sum_n<6>(q, red);
   // du, then dv, then dgd
