@@ -49,6 +49,7 @@ struct gd_handle {
   std::vector<FiltRow> frows;    // filter workgroups: (member, latitude) with staged rows
   FiltRow *dfrows = nullptr;
   double *dtwid = nullptr;       // [N/2+1][2] cos, sin of 2 pi k/N
+  double *dghost = nullptr;      // [2][P]: 1.0 row, 0.0 row (stencil rows beyond the grid)
   int fft_L = 1, fft_half = 1;
   std::vector<int> fft_rad;
   // fused phase launches (k_phase): the filter rows inside the tendency
@@ -111,6 +112,7 @@ Args base_args(const gd_handle *h) {
   a.slots = h->dslots;
   a.frows = h->dfrows;
   a.twid = h->dtwid;
+  a.ghost = h->dghost;
   a.fft_L = h->fft_L;
   a.fft_half = h->fft_half;
   a.fft_nrad = (int)h->fft_rad.size();
@@ -626,8 +628,9 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->dfrows = (FiltRow *)dalloc(h, sizeof(FiltRow) * std::max<size_t>(h->frows.size(), 1), e);
   h->dtwid = (double *)dalloc(h, (size_t)tw_padded(h->N) * 2 * 8, e);
   h->dstep = (int32_t *)dalloc(h, sizeof(int32_t) * h->nmem, e);
+  h->dghost = (double *)dalloc(h, (size_t)2 * h->P * 8, e);
   if (!h->ddiag || !h->dcoef || !h->dslots || !h->dfrows || !h->dtwid ||
-      !h->dstep) {
+      !h->dstep || !h->dghost) {
     gd_destroy(h);
     return fail(nullptr, GD_ERR_HIP, "device allocation failed");
   }
@@ -645,6 +648,9 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   if (!h->frows.empty())
     hipMemcpyAsync(h->dfrows, h->frows.data(), sizeof(FiltRow) * h->frows.size(), hipMemcpyHostToDevice, h->stream);
   hipMemcpyAsync(h->dtwid, tw.data(), tw.size() * 8, hipMemcpyHostToDevice, h->stream);
+  std::vector<double> ghost((size_t)2 * h->P, 0.0);
+  std::fill(ghost.begin(), ghost.begin() + h->P, 1.0);
+  hipMemcpyAsync(h->dghost, ghost.data(), ghost.size() * 8, hipMemcpyHostToDevice, h->stream);
   hipMemsetAsync(h->dstep, 0, sizeof(int32_t) * h->nmem, h->stream);
   e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) {

@@ -68,38 +68,46 @@ __device__ __forceinline__ void seg_load(const Args &a, const Member &mb, int j,
   const double *ghs = mb.ghs();
   auto wrap = [N](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
   const int lcol = tid < W ? i0 + tid : (tid == W ? i0 - 1 : i0 + W);
-  const bool on = tid < W + 2;
   const int p = wrap(lcol), pn = wrap(lcol + 1);
+  // Every lane loads, whatever its column (lanes past the right halo load
+  // its column and are not used), and a row beyond the grid is read from a
+  // ghost row of the value the stencil takes there (gd 1, winds 0).  No load
+  // then sits under a condition: the compiler would make that a branch,
+  // move the value's use into it and wait for each load in turn.
+  const double *one = a.ghost, *zero = a.ghost + P;
+  auto in = [M](int row, int last) { return row >= 0 && row <= last; };
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int row = j - 1 + r;
-    L.lg[r] = on && row >= 0 && row <= M - 1 ? gd[(size_t)row * P + p] : 1.0;
+    L.lg[r] = (in(row, M - 1) ? gd + (size_t)row * P : one)[p];
   }
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int row = j + r;
-    L.lgn[r] = on && row <= M - 1 ? gd[(size_t)row * P + pn] : 1.0;
-    L.lfu[r] = IN_FULL && on && row <= M - 1 ? mb.st(a.s_in, Fu)[(size_t)row * P + p] : 0.0;
+    L.lgn[r] = (in(row, M - 1) ? gd + (size_t)row * P : one)[pn];
+    L.lfu[r] = IN_FULL ? (in(row, M - 1) ? mb.st(a.s_in, Fu) + (size_t)row * P : zero)[p] : 0.0;
   }
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     const int row = j - 1 + r;
-    L.lU[r] = on && row >= 0 && row <= M - 1 ? Ug[(size_t)row * P + p] : 0.0;
-    L.lV[r] = on && row >= 0 && row <= M - 2 ? Vg[(size_t)row * P + p] : 0.0;
-    L.lfv[r] = IN_FULL && on && row >= 0 && row <= M - 2 ? mb.st(a.s_in, Fv)[(size_t)row * P + p] : 0.0;
+    L.lU[r] = (in(row, M - 1) ? Ug + (size_t)row * P : zero)[p];
+    L.lV[r] = (in(row, M - 2) ? Vg + (size_t)row * P : zero)[p];
+    L.lfv[r] = IN_FULL ? (in(row, M - 2) ? mb.st(a.s_in, Fv) + (size_t)row * P : zero)[p] : 0.0;
   }
   // the own point's geopotential: gd at (j, i), (j, i+1), (j+1, i) is the
   // column load already made (lg[1], lgn[0], lg[2]); only ghs is loaded
-  const int i = i0 + tid;
+  // (lanes tid >= W load column i0 and are not used)
+  const int i = tid < W ? i0 + tid : i0;
   const int ip = i + 1 >= N ? i + 1 - N : i + 1;
   const size_t o = (size_t)j * P;
   L.g_i = L.lg[1];
   L.g_ip = L.lgn[0];
   L.g_n = L.lg[2];
   L.h_i = L.h_ip = L.h_n = 0.0;
-  if (tid < W && PASS != 2) {
-    L.h_i = ghs[o + i]; L.h_ip = ghs[o + ip];
-    if (j <= M - 2) L.h_n = ghs[o + P + i];
+  if (PASS != 2) {
+    L.h_i = ghs[o + i];
+    L.h_ip = ghs[o + ip];
+    L.h_n = (j <= M - 2 ? ghs + o + P : zero)[i];
   }
 }
 
@@ -112,11 +120,15 @@ struct RowNbr {
 
 __device__ __forceinline__ RowNbr row_nbr(const Args &a, int j) {
   const int M = a.M;
+  // read through the constant address space (no kernel writes the table):
+  // scalar loads, which the vector-memory waits do not count, so no wait
+  // for them also waits for the loads in flight before them
+  const __attribute__((address_space(4))) RowCoef *coef = (const __attribute__((address_space(4))) RowCoef *)a.coef;
   RowNbr nb;
-  nb.hc_m = j >= 1 ? a.coef[j - 1].hcos : 0.0;
-  nb.hc_p = j + 1 <= M - 2 ? a.coef[j + 1].hcos : 0.0;
-  nb.f_p = j + 1 <= M - 1 ? a.coef[j + 1].f : 0.0;
-  nb.c_p = j + 1 <= M - 1 ? a.coef[j + 1].c : 0.0;
+  nb.hc_m = j >= 1 ? coef[j - 1].hcos : 0.0;
+  nb.hc_p = j + 1 <= M - 2 ? coef[j + 1].hcos : 0.0;
+  nb.f_p = j + 1 <= M - 1 ? coef[j + 1].f : 0.0;
+  nb.c_p = j + 1 <= M - 1 ? coef[j + 1].c : 0.0;
   return nb;
 }
 
