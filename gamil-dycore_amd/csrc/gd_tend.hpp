@@ -37,16 +37,37 @@ constexpr int TC_DU = 1, TC_DV = 2, TC_DGD = 4, TC_RING = 8, TC_ALL = 15;
 
 // Pole mass flux of pole row j (dycore_mod.F90:572-596): the ring sum over
 // the row, in the same order in every caller.  Ends with barriers (block_sum).
-__device__ __forceinline__ double pole_flux(const Args &a, const Member &mb, int j, double *red) {
+// This thread's part of the ring sum of pole row j: longitudes q = tid,
+// tid + kBlock, ... in that order.  The loads go in blocks of four, each
+// issued without a condition: past N a lane reads the ghost rows (gd 1,
+// V 0), whose term (1 + 1) * 0 = +0 leaves the sum as it is.
+__device__ __forceinline__ double ring_part(const Args &a, const Member &mb, int j) {
   const int N = a.N, M = a.M, P = a.pitch, tid = threadIdx.x;
   const double *gd = mb.st(a.s_in, FGD);
   const double *Vg = mb.st(a.s_in, FV);
+  const double *one = a.ghost, *zero = a.ghost + P;
   const int jn = j == 0 ? 1 : M - 2, hr = j == 0 ? 0 : M - 2;
   double pacc = 0.0;
-#pragma unroll 4
-  for (int q = tid; q < N; q += kBlock)
-    pacc += (sqrt(gd[(size_t)j * P + q]) + sqrt(gd[(size_t)jn * P + q])) * Vg[(size_t)hr * P + q];
-  double sum = block_sum(pacc, red);
+  for (int q0 = 0; q0 < N; q0 += 4 * kBlock) {
+    double g0[4], g1[4], v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = q0 + k * kBlock + tid;
+      const bool in = q < N;
+      const int qc = in ? q : 0;
+      g0[k] = (in ? gd + (size_t)j * P : one)[qc];
+      g1[k] = (in ? gd + (size_t)jn * P : one)[qc];
+      v[k] = (in ? Vg + (size_t)hr * P : zero)[qc];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pacc += (sqrt(g0[k]) + sqrt(g1[k])) * v[k];
+  }
+  return pacc;
+}
+
+__device__ __forceinline__ double pole_flux(const Args &a, const Member &mb, int j, double *red) {
+  const int N = a.N, M = a.M;
+  double sum = block_sum(ring_part(a, mb, j), red);
   if (j == M - 1) sum = -sum;
   return sum * 2.0 / N / kRadius / a.dlat;
 }
@@ -75,7 +96,7 @@ __device__ __forceinline__ void seg_load(const Args &a, const Member &mb, int j,
   // then sits under a condition: the compiler would make that a branch,
   // move the value's use into it and wait for each load in turn.
   const double *one = a.ghost, *zero = a.ghost + P;
-  auto in = [M](int row, int last) { return row >= 0 && row <= last; };
+  auto in = [](int row, int last) { return row >= 0 && row <= last; };
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int row = j - 1 + r;
@@ -332,21 +353,13 @@ template <typename CT, int PASS, bool IN_FULL, int CM = TC_ALL>
 __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &mb, const RowCoef &cj, int j,
                                                   int i0, int W, bool stg_u, bool stg_v, bool stg_g,
                                                   unsigned char *smem, double *red) {
-  const int N = a.N, M = a.M, P = a.pitch, tid = threadIdx.x;
-  const double *gd = mb.st(a.s_in, FGD);
-  const double *Vg = mb.st(a.s_in, FV);
+  const int M = a.M, P = a.pitch, tid = threadIdx.x;
   // pole mass flux sums over the whole ring (dycore_mod.F90:572-596); every
-  // segment of a pole row forms the same sum in the same order (that of
-  // pole_flux).  Pole tiles finish these loads before issuing the halo
+  // segment of a pole row forms the same sum in the same order (ring_part,
+  // as pole_flux).  Pole tiles finish these loads before issuing the halo
   // loads, so the two sets of registers are never live together.
   const bool pole_row = (CM & TC_RING) != 0 && PASS != 2 && (j == 0 || j == M - 1);
-  double pacc = 0.0;
-  if (pole_row) {
-    const int jn = j == 0 ? 1 : M - 2, hr = j == 0 ? 0 : M - 2;
-#pragma unroll 4
-    for (int q = tid; q < N; q += kBlock)
-      pacc += (sqrt(gd[(size_t)j * P + q]) + sqrt(gd[(size_t)jn * P + q])) * Vg[(size_t)hr * P + q];
-  }
+  const double pacc = pole_row ? ring_part(a, mb, j) : 0.0;
   const RowNbr nb = row_nbr(a, j);
   SegLoads L;
   seg_load<PASS, IN_FULL>(a, mb, j, i0, W, L);
