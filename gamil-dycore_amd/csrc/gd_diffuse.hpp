@@ -23,16 +23,37 @@ struct DiffPoint {
   double gdd, ud, vd;
 };
 
+// This thread's part of the zonal sum of the gd differences next to pole
+// row j: longitudes q = tid, tid + kBlock, ... in that order, loaded in
+// blocks of four without conditions (past N a lane reads the gd ghost row
+// twice, and 1 - 1 = +0 leaves the sum as it is).
+__device__ __forceinline__ double dif_ring_part(const Args &a, const double *lgd, int j) {
+  const int N = a.N, M = a.M, P = a.pitch, tid = threadIdx.x;
+  const int ja = j == 0 ? 1 : M - 1, jb = j == 0 ? 0 : M - 2;
+  double pacc = 0.0;
+  for (int q0 = 0; q0 < N; q0 += 4 * kBlock) {
+    double x[4], y[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = q0 + k * kBlock + tid;
+      const bool in = q < N;
+      const int qc = in ? q : 0;
+      x[k] = (in ? lgd + (size_t)ja * P : a.ghost)[qc];
+      y[k] = (in ? lgd + (size_t)jb * P : a.ghost)[qc];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pacc += x[k] - y[k];
+  }
+  return pacc;
+}
+
 // Pole-row value of gdd (diffusion_mod.F90:116-133): the zonal sum of the
 // operand's gd differences next to the pole, in the same order in every
 // caller.  Ends with barriers (block_sum).
 __device__ __forceinline__ double dif_pole(const Args &a, const Member &mb, int j, double *red) {
-  const int N = a.N, M = a.M, P = a.pitch, tid = threadIdx.x;
+  const int N = a.N;
   const double *lgd = a.lap_src != 0 ? mb.st(a.s_lap, FGD) : mb.st(a.s_in, FGD);
-  const int ja = j == 0 ? 1 : M - 1, jb = j == 0 ? 0 : M - 2;
-  double pacc = 0.0;
-#pragma unroll 4
-  for (int q = tid; q < N; q += kBlock) pacc += lgd[(size_t)ja * P + q] - lgd[(size_t)jb * P + q];
+  const double pacc = dif_ring_part(a, lgd, j);
   const RowCoef cj = a.coef[j];
   const double s = block_sum(pacc, red);
   if (j == 0) return s * cj.hcos / cj.fdlat2 * cj.fcos / N;
@@ -66,32 +87,31 @@ __device__ __forceinline__ DiffPoint dif_segment(const Args &a, const Member &mb
   auto wrap = [N](int i) { return i < 0 ? i + N : (i >= N ? i - N : i); };
   double lg[4], lgn[3], lU[3], lV[3];
   {
+    // every lane loads (lanes e >= W+2 are not used); a row beyond the grid
+    // reads the ghost row of the value the stencil takes there (gd 1,
+    // winds 0), so no load sits under a condition that would make it wait
+    // alone
+    const double *one = a.ghost, *zero = a.ghost + P;
     const int e = tid;
-    const bool on = e < W + 2;
     const int p = wrap(i0 - 1 + e), pn = wrap(i0 + e);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = j - 1 + r;
-      lg[r] = on && row >= 0 && row <= M - 1 ? lgd[(size_t)row * P + p] : 1.0;
+      lg[r] = (row >= 0 && row <= M - 1 ? lgd + (size_t)row * P : one)[p];
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const int row = j - 1 + r;
-      const bool urow = on && row >= 1 && row <= M - 2;
-      const bool vrow = on && row >= 0 && row <= M - 2;
-      lgn[r] = urow && !full ? gd[(size_t)row * P + pn] : 1.0;
-      lU[r] = urow ? lu[(size_t)row * P + p] : 0.0;
-      lV[r] = vrow ? lv[(size_t)row * P + p] : 0.0;
+      const bool urow = row >= 1 && row <= M - 2;
+      const bool vrow = row >= 0 && row <= M - 2;
+      lgn[r] = (urow && !full ? gd + (size_t)row * P : one)[pn];
+      lU[r] = (urow ? lu + (size_t)row * P : zero)[p];
+      lV[r] = (vrow ? lv + (size_t)row * P : zero)[p];
     }
   }
   // pole rows of gdd: zonal sums (diffusion_mod.F90:116-133)
   const bool pole_row = (CM & TC_RING) != 0 && (j == 0 || j == M - 1);
-  double pacc = 0.0;
-  if (pole_row) {
-    const int ja = j == 0 ? 1 : M - 1, jb = j == 0 ? 0 : M - 2;
-#pragma unroll 4
-    for (int q = tid; q < N; q += kBlock) pacc += lgd[(size_t)ja * P + q] - lgd[(size_t)jb * P + q];
-  }
+  const double pacc = pole_row ? dif_ring_part(a, lgd, j) : 0.0;
   if (tid < W + 2) {
     const int li = tid + 1;
 #pragma unroll
