@@ -181,11 +181,13 @@ void set_fused(const gd_handle *h, Args &a, int nm) {
 }
 
 // Fused phase launches pay off while a launch is a few rounds of resident
-// tiles (latency bound); past about eight rounds the lower occupancy of
-// k_phase (five tiles per CU against seven or eight for k_tend) costs more
-// than the launches it saves.  Measured crossovers on MI355X: 8 to 16 runs
-// of 720x361, 16 to 32 of 360x181, 32 to 128 of 180x91.
-constexpr int64_t kFusedMaxTiles = 10240;
+// tiles (latency bound); past that the lower occupancy of k_phase (five
+// tiles per CU against seven or eight for k_tend) costs more than the
+// launches it saves.  Measured on MI355X (mixed, aggregate model-days/s):
+// fused still ahead at 16 runs of 720x361 (17,328 tiles), 64 members of
+// 360x181 (23,168) and 128 of 180x91 (11,648); even at 32 runs of 720x361
+// (34,656).
+constexpr int64_t kFusedMaxTiles = 24576;
 
 bool use_fused(const gd_handle *h) {
   if (!h->fused_ok || h->fused_mode == 0) return false;

@@ -106,15 +106,15 @@ def test_fused_phase_off_above_1024_longitudes(gd):
 
 def test_fused_phase_rule_by_launch_size(gd):
     """Without GD_FUSED, phases are fused while active members x num_lat x
-    segments <= 10240 row tiles (180x91: 91 tiles per member)."""
+    segments <= 24576 row tiles (360x181: 362 tiles per member)."""
     old = os.environ.pop("GD_FUSED", None)
     try:
-        N, M = 180, 91
-        with gd.Dycore(N, M, 480.0, members=128, zonal_tend_filter_cutoff_wavenumber=gd.cutoff_from_lat0(N, M, 75.0)) as d:
+        N, M = 360, 181
+        with gd.Dycore(N, M, 240.0, members=68, zonal_tend_filter_cutoff_wavenumber=gd.cutoff_from_lat0(N, M, 75.0)) as d:
             assert not d.fused_phases()
-            d.set_active_members(112)
+            d.set_active_members(67)
             assert d.fused_phases()
-            d.set_active_members(113)
+            d.set_active_members(68)
             assert not d.fused_phases()
     finally:
         if old is not None:
