@@ -485,7 +485,8 @@ std::string validate(const gd_config &c, int nmem) {
   if (nmem < 1 || nmem > 4096) return "n_members must be in [1, 4096]";
   if (!(c.time_step_size > 0)) return "time_step_size must be > 0";
   if (c.split_scheme < 0 || c.split_scheme > 3) return "unknown split_scheme";
-  if (c.split_scheme == GD_SPLIT_CSP2 && c.subcycles < 1) return "subcycles must be >= 1";
+  if ((c.split_scheme == GD_SPLIT_CSP2 || c.split_scheme == GD_SPLIT_ISP) && c.subcycles < 1)
+    return "subcycles must be >= 1";
   if (c.uv_adv_scheme < 0 || c.uv_adv_scheme > 2) return "unknown uv_adv_scheme";
   if (c.precision < 0 || c.precision > 2) return "unknown precision";
   return "";
@@ -529,6 +530,8 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   *out = nullptr;
   std::string v = validate(*cfg, n_members);
   if (!v.empty()) return fail(nullptr, GD_ERR_INVALID_ARG, v);
+  if (cfg->precision == GD_PREC_MIXED_BF16)
+    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "precision 'mixed_bf16' is not provided");
   if (cfg->uv_adv_scheme == GD_ADV_WENO)
     return fail(nullptr, GD_ERR_NOT_SUPPORTED, "uv_adv_scheme 'weno' is not provided on the device path");
   if (cfg->use_diffusion && cfg->diffusion_order < 2)
@@ -785,6 +788,8 @@ int gd_space_operators(gd_handle *h, int32_t member, int32_t pass, double *du, d
   int rc = check_member(h, member);
   if (rc) return rc;
   if (pass < 0 || pass > 2 || ld < h->N) return fail(h, GD_ERR_INVALID_ARG, "bad pass or ld");
+  // the launch evaluates members 0..nact-1 only
+  if (member >= h->nact) return fail(h, GD_ERR_INVALID_ARG, "member is not active");
   hipSetDevice(h->device);
   Args a = base_args(h);
   a.pass = pass;
@@ -803,9 +808,10 @@ int gd_predict_correct(gd_handle *h, double dt, int32_t pass, int32_t qcon, doub
   if (!h || pass < 0 || pass > 2) return GD_ERR_INVALID_ARG;
   hipSetDevice(h->device);
   GD_HIP(h, enqueue_call(h, dt, pass, SX, SY, qcon));
-  GD_HIP(h, enqueue_finish(h, 1, SY, 0, h->nmem));
+  // only the active members were stepped: the others keep their state
+  GD_HIP(h, enqueue_finish(h, 1, SY, 0, h->nact));
   if (beta)
-    for (int m = 0; m < h->nmem; ++m)
+    for (int m = 0; m < h->nact; ++m)
       GD_HIP(h, hipMemcpyAsync(beta + m, hmember(h, m).beta(), 8, hipMemcpyDeviceToHost, h->stream));
   GD_HIP(h, hipStreamSynchronize(h->stream));
   return GD_OK;
@@ -816,11 +822,11 @@ int gd_ordinary_diffusion(gd_handle *h, double dt) {
   if (h->cfg.diffusion_order < 2) return fail(h, GD_ERR_INVALID_ARG, "diffusion_order must be >= 2");
   hipSetDevice(h->device);
   const size_t fb = (size_t)h->M * h->P * 8;
-  for (int m = 0; m < h->nmem; ++m)
+  for (int m = 0; m < h->nact; ++m)
     for (int f = 0; f < NFLD; ++f)
       GD_HIP(h, hipMemcpyAsync(hmember(h, m).st(SY, f), hmember(h, m).st(SX, f), fb, hipMemcpyDeviceToDevice, h->stream));
   GD_HIP(h, enqueue_diffusion(h, SY, true, dt));
-  GD_HIP(h, enqueue_finish(h, 0, SX, 0, h->nmem));
+  GD_HIP(h, enqueue_finish(h, 0, SX, 0, h->nact));
   GD_HIP(h, hipStreamSynchronize(h->stream));
   return GD_OK;
 }

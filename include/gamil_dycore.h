@@ -49,7 +49,7 @@ enum {
   GD_PREC_MIXED = 1,       /* FP32 momentum stencils; FP64 state update, stored
                               tendencies, pole sums, mass flux divergence, polar
                               filter and integrals                               */
-  GD_PREC_MIXED_BF16 = 2   /* accepted for namelist compatibility; runs as MIXED   */
+  GD_PREC_MIXED_BF16 = 2   /* reserved: gd_create returns GD_ERR_NOT_SUPPORTED     */
 };
 
 /* &dycore_params (src/params_mod.F90:13-98) plus the device knobs. */
@@ -131,13 +131,15 @@ int gd_get_diag_fields(gd_handle *h, int32_t member, double *div, double *vor,
 
 /* space_operators(state(old), tend, pass) (dycore_mod.F90:184-365) including
  * the polar filter and its inner-product rescale; du (num_lat rows, pole rows
- * 0), dv (num_lat-1 rows), dgd (num_lat rows).  The state is not changed. */
+ * 0), dv (num_lat-1 rows), dgd (num_lat rows).  The state is not changed.
+ * member must be active (GD_ERR_INVALID_ARG otherwise). */
 int gd_space_operators(gd_handle *h, int32_t member, int32_t pass, double *du,
                        double *dv, double *dgd, int64_t ld);
 
 /* One integrator call predict_correct(dt, old, new, pass)
- * (dycore_mod.F90:754-792) whose result becomes the current state; beta
- * (n_members values, may be NULL) as logged at dycore_mod.F90:787. */
+ * (dycore_mod.F90:754-792) whose result becomes the current state of the
+ * active members; beta (n_active values, may be NULL) as logged at
+ * dycore_mod.F90:787. */
 int gd_predict_correct(gd_handle *h, double dt, int32_t pass, int32_t qcon_modified,
                        double *beta);
 

@@ -183,3 +183,32 @@ def test_active_members_and_member_copy(gd):
             np.testing.assert_array_equal(a, b)
         with pytest.raises(gd.DycoreError):
             e.set_active_members(4)
+
+
+def test_seams_leave_inactive_members_alone(gd):
+    """predict_correct and ordinary_diffusion step the active members only:
+    an inactive member keeps its state bit for bit, and space_operators
+    refuses an inactive member instead of returning a stale tendency."""
+    N, M = 72, 37
+    kw = _cfg(gd, N, M, subcycles=3, diffusion_coef=2e5)
+    print("seams with 3 members, 2 active", flush=True)
+    with gd.Dycore(N, M, 600.0, members=3, **kw) as d, gd.Dycore(N, M, 600.0, **kw) as one:
+        ics = [gd.ic_balanced(N, M, seed=31 + m, nmax=6) for m in range(3)]
+        for m, ic in enumerate(ics):
+            d.set_state(*ic, member=m)
+        one.set_state(*ics[1])
+        before = d.get_state(member=2)
+        d.set_active_members(2)
+        beta = d.predict_correct(300.0, "all")
+        d.ordinary_diffusion(300.0)
+        for x, y in zip(d.get_state(member=2), before):
+            np.testing.assert_array_equal(x, y)
+        # an active member advanced exactly as a single run does
+        b1 = one.predict_correct(300.0, "all")
+        one.ordinary_diffusion(300.0)
+        assert beta[1] == b1[0]
+        for x, y in zip(d.get_state(member=1), one.get_state()):
+            np.testing.assert_array_equal(x, y)
+        with pytest.raises(gd.DycoreError, match="not active"):
+            d.space_operators("all", member=2)
+        d.space_operators("all", member=1)

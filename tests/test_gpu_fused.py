@@ -71,6 +71,10 @@ CASES = [
     (45, 23, 900.0, 4, dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_coef=2e5)),
     (360, 181, 240.0, 3, dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=1e5)),
     (1024, 33, 30.0, 2, dict(split_scheme="csp2", subcycles=2, use_diffusion=True, diffusion_coef=1e4)),
+    # the benchmarked shape (compile-time 360-point FFT plan with the radix-8
+    # swizzle, three segments per row) and the 180-longitude plan
+    (720, 361, 120.0, 3, dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=2.5e4)),
+    (180, 91, 480.0, 3, dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=1e5)),
     (72, 37, 900.0, 4, dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_order=4,
                             diffusion_coef=2e5 * 4e10)),
 ]

@@ -84,7 +84,11 @@ def test_space_operators_unfiltered_rows_bitwise(gd, oracle_mod):
 def test_filter_rows(gd, oracle_mod, N, M, cut):
     o, d = _pair(gd, oracle_mod, N, M, 600.0, cutoff=cut)
     rng = np.random.default_rng(1)
-    ff, hf, _, _ = d.filter_table()
+    # the device's flag and cutoff table is filter_init's (filter_mod.F90:35-103)
+    for x, y in zip(d.filter_table(), o.filter_table()):
+        np.testing.assert_array_equal(x, y)
+    # the rows to compare are chosen by the oracle's flags
+    ff, hf, _, _ = o.filter_table()
     x = rng.standard_normal((M, N))
     y = d.filter_rows(x, half=False)
     for j in range(M):
