@@ -15,7 +15,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "_build", "libgd_oracle.so")
+# GD_ORACLE_LIB points the tests at another build of the restatement, e.g. a
+# scratch build with a deliberate slip, to show that the pins catch it
+_LIB = os.environ.get("GD_ORACLE_LIB") or os.path.join(_HERE, "_build", "libgd_oracle.so")
 _LIB_FAST = os.path.join(_HERE, "_build", "libgd_oracle_fast.so")
 _REF_FFT = os.path.join(_HERE, "_ref", "libfftpack_ref.so")
 
