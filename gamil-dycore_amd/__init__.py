@@ -93,7 +93,7 @@ EXPORTS = (
     "gd_device_bytes", "gd_cutoff_from_lat0", "gd_time_kernel", "gd_history_write",
     "gd_restart_write", "gd_restart_read", "gd_io_last_error", "gd_ic_steady_geostrophic",
     "gd_ic_jet_zonal_flow", "gd_ic_shallow_water_waves", "gd_serve", "gd_serve_last_error",
-    "gd_set_active_members", "gd_copy_member", "gd_fused_phases",
+    "gd_set_active_members", "gd_copy_member", "gd_fused_phases", "gd_filter_rows_noalloc",
 )
 
 _lib = None
@@ -125,6 +125,7 @@ def lib():
     L.gd_ordinary_diffusion.argtypes = [vp, C.c_double]
     L.gd_filter_rows.argtypes = [vp, C.c_int32, dp, i64]
     L.gd_filter_table.argtypes = [vp, ip, ip, ip, ip]
+    L.gd_filter_rows_noalloc.argtypes = [vp, C.c_int32, C.c_void_p, i64, vp]
     L.gd_ic_balanced.argtypes = [C.c_int32, C.c_int32, C.POINTER(ICParams), dp, dp, dp, dp, i64]
     L.gd_ic_rossby_haurwitz.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double,
                                         C.c_double, dp, dp, dp, dp, i64]
@@ -436,6 +437,13 @@ class Dycore:
         y = np.ascontiguousarray(x, dtype=np.float64).copy()
         _check(lib().gd_filter_rows(self.h, int(bool(half)), _dp(y), self.N), self.h)
         return y
+
+    def filter_rows_noalloc(self, x_ptr, half=False, ld=None, stream=None):
+        """Filter the flagged rows of a float64 array in device memory at
+        address x_ptr in place (gd_filter_rows_noalloc: no allocation; with
+        stream=None the call waits for the result)."""
+        _check(lib().gd_filter_rows_noalloc(self.h, int(bool(half)), C.c_void_p(int(x_ptr)),
+                                            int(self.N if ld is None else ld), stream), self.h)
 
     def filter_table(self):
         ff = np.zeros(self.M, np.int32); hf = np.zeros(self.M - 1, np.int32)

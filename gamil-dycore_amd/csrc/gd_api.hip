@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,15 @@ using namespace gd;
 namespace {
 
 thread_local std::string g_create_error;
+
+// Staging rows of gd_filter_rows_noalloc: one array per device, resolved to
+// its device address once per device (g_scratch_ptr) when the first handle on
+// that device is created.  In host code the name `scratch` is only a host-side
+// stand-in for the symbol; kernels and copies use the resolved address.
+constexpr size_t kScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;
 
 struct ProfEntry {
   std::string family;
@@ -66,6 +76,13 @@ struct gd_handle {
   bool graph_failed = false;
   std::string err;
   std::vector<ProfEntry> *prof = nullptr;
+  // gd_filter_rows_noalloc: this device's scratch array, and per row kind
+  // (0 full rows, 1 half rows) member 0's filtered rows in staging order with
+  // their filter workgroups (slot = position in the scratch array)
+  double *scratch = nullptr;
+  std::string scratch_err;
+  std::vector<int> prow[2];
+  FiltRow *dprow[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -565,6 +582,22 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
     delete h;
     return fail(nullptr, GD_ERR_HIP, "hipGetDeviceProperties failed");
   }
+  {
+    // the scratch array's address on this device, resolved once per device
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(h->device);
+    if (it == g_scratch_ptr.end()) {
+      void *sp = nullptr;
+      const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+      if (se == hipSuccess && sp) {
+        it = g_scratch_ptr.emplace(h->device, (double *)sp).first;
+      } else {
+        h->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+        hipGetLastError();
+      }
+    }
+    if (it != g_scratch_ptr.end()) h->scratch = it->second;
+  }
   std::vector<RowCoef> rc;
   build_tables(h, rc);
   h->fft_half = h->N % 2 == 0;
@@ -634,8 +667,21 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->dtwid = (double *)dalloc(h, (size_t)tw_padded(h->N) * 2 * 8, e);
   h->dstep = (int32_t *)dalloc(h, sizeof(int32_t) * h->nmem, e);
   h->dghost = (double *)dalloc(h, (size_t)2 * h->P * 8, e);
+  // filter workgroups of gd_filter_rows_noalloc: member 0's rows of one kind
+  // in row order, staged at consecutive rows of the scratch array
+  std::vector<FiltRow> prow_tab[2];
+  for (const FiltRow &f : h->frows) {
+    if (f.member != 0 || f.comp == CDU) continue;
+    const int kind = f.comp == CDV ? 1 : 0;
+    FiltRow g = f;
+    g.slot = (int)h->prow[kind].size();
+    h->prow[kind].push_back(f.row);
+    prow_tab[kind].push_back(g);
+  }
+  for (int k = 0; k < 2; ++k)
+    h->dprow[k] = (FiltRow *)dalloc(h, sizeof(FiltRow) * std::max<size_t>(prow_tab[k].size(), 1), e);
   if (!h->ddiag || !h->dcoef || !h->dslots || !h->dfrows || !h->dtwid ||
-      !h->dstep || !h->dghost) {
+      !h->dstep || !h->dghost || !h->dprow[0] || !h->dprow[1]) {
     gd_destroy(h);
     return fail(nullptr, GD_ERR_HIP, "device allocation failed");
   }
@@ -653,6 +699,10 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   if (!h->frows.empty())
     hipMemcpyAsync(h->dfrows, h->frows.data(), sizeof(FiltRow) * h->frows.size(), hipMemcpyHostToDevice, h->stream);
   hipMemcpyAsync(h->dtwid, tw.data(), tw.size() * 8, hipMemcpyHostToDevice, h->stream);
+  for (int k = 0; k < 2; ++k)
+    if (!prow_tab[k].empty())
+      hipMemcpyAsync(h->dprow[k], prow_tab[k].data(), sizeof(FiltRow) * prow_tab[k].size(), hipMemcpyHostToDevice,
+                     h->stream);
   std::vector<double> ghost((size_t)2 * h->P, 0.0);
   std::fill(ghost.begin(), ghost.begin() + h->P, 1.0);
   hipMemcpyAsync(h->dghost, ghost.data(), ghost.size() * 8, hipMemcpyHostToDevice, h->stream);
@@ -852,6 +902,47 @@ int gd_filter_rows(gd_handle *h, int32_t half, double *x, int64_t ld) {
                         hipMemcpyDeviceToHost));
   }
   (void)rowb;
+  return GD_OK;
+}
+
+int gd_filter_rows_noalloc(gd_handle *h, int32_t half, double *x, int64_t ld, void *stream) {
+  if (!h) return GD_ERR_INVALID_ARG;
+  if (!x || ld < h->N || (half != 0 && half != 1)) return fail(h, GD_ERR_INVALID_ARG, "bad filter_rows arguments");
+  if (!h->scratch) return fail(h, GD_ERR_HIP, "no scratch array on this device: " + h->scratch_err);
+  const std::vector<int> &rows = h->prow[half];
+  const size_t need = rows.size() * (size_t)h->P * sizeof(double);
+  if (need > kScratchBytes)
+    return fail(h, GD_ERR_INVALID_ARG,
+                "gd_filter_rows_noalloc: " + std::to_string(rows.size()) + " filtered rows of " +
+                    std::to_string(h->P) + " doubles need " + std::to_string(need) + " bytes of scratch, more than " +
+                    std::to_string(kScratchBytes));
+  if (rows.empty()) return GD_OK;
+  hipSetDevice(h->device);
+  hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+  const size_t P = h->P, N = h->N;
+  // the flagged rows, one copy per run of consecutive rows, into consecutive
+  // scratch rows and back
+  auto copy_runs = [&](bool in) -> hipError_t {
+    for (size_t k = 0; k < rows.size();) {
+      size_t n = 1;
+      while (k + n < rows.size() && rows[k + n] == rows[k] + (int)n) ++n;
+      double *sp = h->scratch + k * P, *xp = x + (size_t)rows[k] * ld;
+      hipError_t e = in ? hipMemcpy2DAsync(sp, P * 8, xp, (size_t)ld * 8, N * 8, n, hipMemcpyDeviceToDevice, s)
+                        : hipMemcpy2DAsync(xp, (size_t)ld * 8, sp, P * 8, N * 8, n, hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return e;
+      k += n;
+    }
+    return hipSuccess;
+  };
+  GD_HIP(h, copy_runs(true));
+  Args a = base_args(h);
+  a.phase = PH_P;
+  a.sx_ext = h->scratch;
+  a.frows = h->dprow[half];
+  a.ntiles = (int)rows.size();
+  GD_HIP(h, launch_filter(a, h->prec, s));
+  GD_HIP(h, copy_runs(false));
+  if (!stream) GD_HIP(h, hipStreamSynchronize(s));
   return GD_OK;
 }
 

@@ -108,6 +108,8 @@ struct Args {
   const FiltRow *frows;
   const double *twid;          // [N/2+1][2]: cos, sin of 2 pi k / N
   const double *ghost;         // [2][pitch]: a row of 1.0 and a row of 0.0, read for rows beyond the grid
+  double *sx_ext;              // k_filter projection (PH_P): staged rows at sx_ext + slot * pitch
+                               // instead of the member's staging rows (nullptr: the member's)
   int fft_L;                   // complex FFT length: N/2 (even N) or N
   int fft_half;                // 1: even N, real FFT through a half-length complex FFT
   int fft_nrad;                // radix passes

@@ -324,7 +324,8 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
   // inputs (state s0 or Tb), and for a staged row the row itself, all loads
   // issued together; gd + ghs is added after the forward transform, when
   // the loads have long arrived
-  const double *xp = mb.sx() + (size_t)fr.slot * P;
+  double *const sxb = a.sx_ext ? a.sx_ext : mb.sx();
+  const double *xp = sxb + (size_t)fr.slot * P;
   const double *wp = mb.st(a.s_in, comp == CDU ? FU : (comp == CDV ? FV : FGD)) + o;
   const bool gdrow = comp == CDGD;
   const double *wq = mb.ghs() + o;
@@ -430,7 +431,7 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
         mb.st(SX, Fv)[o + l] = v + a.kdiff * T;
       }
     } else {  // PH_P: projection only, back into the staging row
-      mb.sx()[(size_t)fr.slot * P + l] = T;
+      sxb[(size_t)fr.slot * P + l] = T;
     }
   }
   if (a.phase == PH_C) {

@@ -152,6 +152,19 @@ int gd_ordinary_diffusion(gd_handle *h, double dt);
  * unflagged rows are left as they are).  Runs the device filter kernel. */
 int gd_filter_rows(gd_handle *h, int32_t half, double *x, int64_t ld);
 
+/* No counterpart in the reference.  gd_filter_rows for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): x is in DEVICE memory (num_lat rows for
+ * half = 0, num_lat-1 rows for half = 1, ld elements apart) and is filtered in
+ * place.  The flagged rows of member 0's table are staged in a fixed 4 MiB
+ * array of the library on the handle's device, so the call allocates nothing;
+ * it returns GD_ERR_INVALID_ARG when flagged rows x pitch (num_lon rounded up
+ * to 32) x 8 bytes exceed 4 MiB.  stream: a hipStream_t to enqueue on,
+ * returning without waiting; NULL runs on the handle's stream and waits.
+ * Every call on one device shares that array: calls that overlap on one
+ * device (other streams, other threads, other handles) race. */
+int gd_filter_rows_noalloc(gd_handle *h, int32_t half, double *x, int64_t ld, void *stream);
+
 /* Per-row filter table as set up by filter_init (filter_mod.F90:35-103):
  * flag (0/1) and kept cutoff wavenumber c (-1 when the row has no mask) for
  * full rows (num_lat) and half rows (num_lat-1). */
