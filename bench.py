@@ -31,6 +31,12 @@ IC = dict(seed=20241, nmax=20, slope=-3.0, rms_height=150.0, mean_depth=8000.0,
           n_mountains=3, mountain_seed=777)
 MODEL_DAYS = 10
 
+# The 64-member 1-degree ensemble (5 model days in the full workload): a
+# common balanced state plus a seeded rms-5 m balanced perturbation per member.
+ENS = dict(N=360, M=181, dt=240.0, coef=1.0e5, members=64,
+           ic=dict(seed=20243, nmax=20, slope=-3.0, rms_height=150.0, mean_depth=8000.0, pert_nmax=20,
+                   pert_rms=5.0))
+
 
 def tend_launch_bytes(N, M, prec):
     """Algorithmic HBM bytes of one launch of the tendency kernel in the
@@ -171,27 +177,113 @@ def _tend_roofline(N, M, members, us, fused):
             "avg_launch_us": round(us, 3)}
 
 
+def _ensemble_ics(gd, N, M, members):
+    """The members' initial states, generated on host threads (the generator
+    is host code that releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(lambda k: gd.ic_balanced(N, M, **ENS["ic"], pert_seed=5000 + k), range(members)))
+
+
+def ensemble_workload(gd, steps, warmup, tune=True):
+    """The 64-member 360x181 ensemble (dt 240 s, diffusion 1e5): member-days/s
+    in mixed precision and FP64, launch shapes from the autotuner (which only
+    picks shapes that give the same bits), and the roofline of the dominant
+    launch over all members."""
+    N, M, dt, mem = ENS["N"], ENS["M"], ENS["dt"], ENS["members"]
+    ics = _ensemble_ics(gd, N, M, mem)
+    out, kern, tuned = {}, {}, {}
+    for prec in ("mixed", "fp64"):
+        d = _cfg(gd, N, M, dt, ENS["coef"], prec, members=mem)
+        for k, ic in enumerate(ics):
+            d.set_state(*ic, member=k)
+        if tune and hasattr(d, "autotune"):
+            tuned[prec] = d.autotune(steps=min(40, steps))
+            for k, ic in enumerate(ics):
+                d.set_state(*ic, member=k)
+        el = _time_steps(d, steps, warmup)
+        kern[prec] = _kernel_us(d, 50)
+        d.close()
+        out[prec] = mem * steps * dt / 86400.0 / el
+    return out, _tend_roofline(N, M, mem, *kern["mixed"]), tuned
+
+
+def w1440_workload(gd, steps, warmup):
+    """The 1440x721 single run (dt 60 s, spectrum to n = 80, diffusion
+    6.25e3): model-days/s in mixed precision and FP64 and the roofline of the
+    tendency launch."""
+    N, M, dt = 1440, 721, 60.0
+    ic = gd.ic_balanced(N, M, seed=20242, nmax=80)
+    out, kern = {}, {}
+    for prec in ("mixed", "fp64"):
+        d = _cfg(gd, N, M, dt, 6.25e3, prec)
+        d.set_state(*ic)
+        el = _time_steps(d, steps, warmup)
+        kern[prec] = _kernel_us(d, 100)
+        d.close()
+        out[prec] = steps * dt / 86400.0 / el
+    return out, _tend_roofline(N, M, 1, *kern["mixed"])
+
+
+def mm_flops(d, members):
+    """Multiply-add flops of one matrix-core filter launch: for every
+    momentum row (du on flagged full rows between the poles, dv on flagged
+    half rows) of every member, two products of 2 N K flops (K = 2c + 2
+    kept basis functions).  Returns (algorithmic, executed) where executed
+    counts the padded tiles (N and K up to multiples of 32, members up to
+    multiples of 16) as the kernel runs them."""
+    N, M = d.N, d.M
+    ff, hf, fc, hc = d.filter_table()
+    ks = [2 * int(fc[j]) + 2 for j in range(1, M - 1) if ff[j]] + [2 * int(hc[j]) + 2 for j in range(M - 1) if hf[j]]
+    pad = lambda x: (x + 31) // 32 * 32
+    alg = sum(2 * 2 * N * k for k in ks) * members
+    exe = sum(2 * 2 * pad(N) * pad(k) for k in ks) * ((members + 15) // 16 * 16)
+    return alg, exe
+
+
+def filter_crossover(gd, ens_steps, w1440_steps):
+    """The polar filter's implementations side by side: the FP64 LDS FFT
+    (mixed), FP32 matrix-core products (mixed_mfma) and BF16 split-operand
+    products (mixed_bf16), on the 64-member 360x181 ensemble and the
+    1440x721 single run; with the matrix-core launch's duration and its
+    achieved TFLOP/s against the dense MFMA peaks (FP32 157.3, BF16 2500)."""
+    out = {}
+    N, M, dt, mem = ENS["N"], ENS["M"], ENS["dt"], ENS["members"]
+    cases = [("ensemble_64x360x181", N, M, dt, ENS["coef"], mem, ens_steps, _ensemble_ics(gd, N, M, mem)),
+             ("1440x721", 1440, 721, 60.0, 6.25e3, 1, w1440_steps, [gd.ic_balanced(1440, 721, seed=20242, nmax=80)])]
+    for name, n, m, step_dt, coef, members, steps, ics in cases:
+        row = {}
+        for prec in ("mixed", "mixed_mfma", "mixed_bf16"):
+            d = _cfg(gd, n, m, step_dt, coef, prec, members=members)
+            for k, ic in enumerate(ics):
+                d.set_state(*ic, member=k)
+            el = _time_steps(d, steps, 12)
+            r = {"days_per_s": round(members * steps * step_dt / 86400.0 / el, 4),
+                 "ms_per_step": round(el / steps * 1e3, 4)}
+            if prec != "mixed":
+                us = d.time_kernel("filter_mm", 20)
+                alg, exe = mm_flops(d, members)
+                peak = 157.3 if prec == "mixed_mfma" else 2500.0
+                mult = 1 if prec == "mixed_mfma" else 3  # hi.hi + hi.lo + lo.hi
+                r["filter_mm_us"] = round(us, 2)
+                r["filter_mm_tflops_algorithmic"] = round(alg / (us * 1e-6) / 1e12, 3)
+                r["filter_mm_tflops_executed"] = round(mult * exe / (us * 1e-6) / 1e12, 3)
+                r["mfma_peak_tflops"] = peak
+                r["mfma_frac_executed"] = round(mult * exe / (us * 1e-6) / 1e12 / peak, 4)
+            d.close()
+            row[prec] = r
+        out[name] = row
+    return out
+
+
 def other_workload(gd, args):
     base = {"n_gpus": 1, "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None,
             "dtype": "f32/f64", "roofline": None, "cpu_baseline": None}
     if args.workload == "ensemble":
-        # 64 members at 360x181, dt 240 s, 5 model days (1800 steps); members
-        # share the topography-free balanced state plus their own rms-5 m
-        # perturbation
-        N, M, dt, mem = 360, 181, 240.0, 64
         steps = min(args.steps, 1800)
-        ics = [gd.ic_balanced(N, M, seed=20243, nmax=20, pert_seed=5000 + k, pert_nmax=20, pert_rms=5.0)
-               for k in range(mem)]
-        out, tend_us = {}, {}
-        for prec in ("mixed", "fp64"):
-            d = _cfg(gd, N, M, dt, 1.0e5, prec, members=mem)
-            for k, ic in enumerate(ics):
-                d.set_state(*ic, member=k)
-            el = _time_steps(d, steps, args.warmup)
-            tend_us[prec] = _kernel_us(d, 50)
-            d.close()
-            out[prec] = mem * steps * dt / 86400.0 / el
-        base["roofline"] = _tend_roofline(N, M, mem, *tend_us["mixed"])
+        out, roof, tuned = ensemble_workload(gd, steps, args.warmup)
+        mem, dt = ENS["members"], ENS["dt"]
+        base["roofline"] = roof
         return dict(base, metric="member-days/sec, 64-member ensemble at 360x181 (mixed vs FP64)",
                     value=round(out["mixed"], 3), unit="member-days/s", steps=steps,
                     ms_per_step=round(mem * steps * dt / 86400.0 / out["mixed"] / steps * 1e3, 4),
@@ -199,23 +291,14 @@ def other_workload(gd, args):
                     config={"workload": "ensemble 64 x 360x181, dt=240 s, csp2 x6, diffusion 1e5, "
                                         f"filter cos(lat)/cos({FILTER_LAT0:g})",
                             "fp64_member_days_per_s": round(out["fp64"], 3),
-                            "mixed_over_fp64": round(out["mixed"] / out["fp64"], 3)})
+                            "mixed_over_fp64": round(out["mixed"] / out["fp64"], 3), "autotuned": tuned})
     if args.workload == "1440":
-        N, M, dt = 1440, 721, 60.0
         steps = min(args.steps, 2880)
-        ic = gd.ic_balanced(N, M, seed=20242, nmax=80)
-        out, tend_us = {}, {}
-        for prec in ("mixed", "fp64"):
-            d = _cfg(gd, N, M, dt, 6.25e3, prec)
-            d.set_state(*ic)
-            el = _time_steps(d, steps, args.warmup)
-            tend_us[prec] = _kernel_us(d, 100)
-            d.close()
-            out[prec] = steps * dt / 86400.0 / el
-        base["roofline"] = _tend_roofline(N, M, 1, *tend_us["mixed"])
+        out, roof = w1440_workload(gd, steps, args.warmup)
+        base["roofline"] = roof
         return dict(base, metric="model-days/sec at 1440x721 (mixed vs FP64)",
                     value=round(out["mixed"], 4), unit="model-days/s", steps=steps,
-                    ms_per_step=round(dt / 86400.0 / out["mixed"] * 1e3, 4), scaling="weak",
+                    ms_per_step=round(60.0 / 86400.0 / out["mixed"] * 1e3, 4), scaling="weak",
                     data="synthetic (seeded balanced state, n <= 80)",
                     config={"workload": "1440x721 (0.25 deg), dt=60 s, csp2 x6, diffusion 6.25e3, "
                                         f"filter cos(lat)/cos({FILTER_LAT0:g})",
@@ -311,6 +394,70 @@ def cpu_baseline(ic, nsteps):
     return nsteps * WORKLOAD["time_step_size"] / 86400.0 / el, el
 
 
+def cpu_info():
+    """Host CPU of this run: logical CPUs and the model name."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def cpu_worker(args):
+    """One CPU process of the ensemble baseline: the -Ofast FP64 oracle, one
+    thread, `cpu_worker_steps` steps of one member of the 64-member 360x181
+    ensemble (state from the npz its parent wrote); prints the seconds."""
+    from oracle import oracle as orc
+    orc.use_timing_build()
+    orc.set_threads(1)
+    f = np.load(args.cpu_worker)
+    N, M, dt = ENS["N"], ENS["M"], ENS["dt"]
+    o = orc.Oracle(N, M, dt, split_scheme="csp2", subcycles=6, qcon_modified=True,
+                   uv_adv_scheme="center_diff", use_diffusion=True, diffusion_order=2,
+                   diffusion_coef=ENS["coef"], use_zonal_tend_filter=True, cutoff=list(f["cut"]))
+    o.set_initial_state(f["u"], f["v"], f["gd"], f["ghs"])
+    o.start()
+    t0 = time.perf_counter()
+    o.run(args.cpu_worker_steps)
+    print(f"{time.perf_counter() - t0:.6f}", flush=True)
+
+
+def cpu_ensemble_baseline(gd, nproc, steps):
+    """P independent single-thread processes of the -Ofast oracle (the
+    reference is serial), each stepping its own ensemble member: aggregate
+    member-days/s.  Run before the process touches the GPU."""
+    import subprocess
+    import tempfile
+    N, M = ENS["N"], ENS["M"]
+    cut = gd.cutoff_from_lat0(N, M, FILTER_LAT0)
+    procs, files = [], []
+    tmp = tempfile.mkdtemp(prefix="gd_cpu_")
+    for k in range(nproc):
+        u, v, h, ghs = gd.ic_balanced(N, M, **ENS["ic"], pert_seed=5000 + k)
+        path = os.path.join(tmp, f"m{k}.npz")
+        np.savez(path, u=u, v=v, gd=h, ghs=ghs, cut=np.array(cut, np.int32))
+        files.append(path)
+    t0 = time.perf_counter()
+    for path in files:
+        procs.append(subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", path,
+                                       "--cpu-worker-steps", str(steps)], stdout=subprocess.PIPE, text=True))
+    secs = []
+    for pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            raise RuntimeError("CPU baseline worker failed")
+        secs.append(float(out.strip().splitlines()[-1]))
+    wall = time.perf_counter() - t0
+    for path in files:
+        os.remove(path)
+    os.rmdir(tmp)
+    return nproc * steps * ENS["dt"] / 86400.0 / max(secs), max(secs), wall
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,7 +476,23 @@ def main():
     ap.add_argument("--pmc-probe", type=int, default=0, metavar="K",
                     help="counter runs: one 256 MiB device copy (calibration of the byte "
                          "counters) then K steps of the mixed workload, no timing output")
+    ap.add_argument("--mm-probe", type=int, default=0, metavar="K",
+                    help="counter runs: K steps of the 64-member ensemble in the two matrix-core "
+                         "filter modes, no timing output")
+    ap.add_argument("--no-extra", dest="extra", action="store_false",
+                    help="skip the short ensemble / 1440x721 / serving measurements of the default line")
+    ap.add_argument("--ens-steps", type=int, default=360)
+    ap.add_argument("--w1440-steps", type=int, default=480)
+    ap.add_argument("--serve-short", type=int, default=12)
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="processes of the CPU ensemble baseline (0: min(16, CPUs))")
+    ap.add_argument("--cpu-ens-steps", type=int, default=24)
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-worker-steps", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(sharding across GPUs is designed, not built)"}))
@@ -341,9 +504,41 @@ def main():
     if args.pmc_probe:
         pmc_probe(gd, ic, args.pmc_probe)
         return
+    if args.mm_probe:
+        # counter runs of the matrix-core filter: K steps of the 64-member
+        # ensemble in each matrix-core mode (run with GD_NO_GRAPH=1 so each
+        # kernel is its own dispatch)
+        ics = _ensemble_ics(gd, ENS["N"], ENS["M"], ENS["members"])
+        for prec in ("mixed_mfma", "mixed_bf16"):
+            d = _cfg(gd, ENS["N"], ENS["M"], ENS["dt"], ENS["coef"], prec, members=ENS["members"])
+            for k, x in enumerate(ics):
+                d.set_state(*x, member=k)
+            d.step(args.mm_probe)
+            alg, exe = mm_flops(d, ENS["members"])
+            d.close()
+            print(f"mm probe {prec}: {args.mm_probe} steps, {alg} algorithmic / {exe} executed flops per "
+                  f"filter launch (x3 products in mixed_bf16)", flush=True)
+        return
     if args.workload != "cfg2":
         print(json.dumps(other_workload(gd, args)))
         return
+    # CPU legs first, before this process touches the GPU (the ensemble leg
+    # starts child processes)
+    cpu = None
+    if not args.no_cpu:
+        cv, cel = cpu_baseline(ic, args.cpu_steps)
+        nproc = args.cpu_procs or min(16, os.cpu_count() or 1)
+        ev, esec, ewall = cpu_ensemble_baseline(gd, nproc, args.cpu_ens_steps)
+        cpu = {"value": round(cv, 6), "unit": "model-days/s", "cores": 1, "kind": "port",
+               "sample": f"first {args.cpu_steps} steps of the same 720x361 FP64 workload, "
+                         f"oracle/gd_oracle.c built -O3 -ffast-math (the reference's -Ofast), "
+                         f"single thread ({cel:.1f} s)",
+               "host": cpu_info(),
+               "ensemble": {"value": round(ev, 5), "unit": "member-days/s", "processes": nproc,
+                            "cores": nproc,
+                            "sample": f"{nproc} single-thread processes, each the first "
+                                      f"{args.cpu_ens_steps} steps of its own member of the 64-member "
+                                      f"360x181 ensemble (same -Ofast oracle; slowest {esec:.1f} s)"}}
     res = {}
     for prec in ("mixed", "fp64"):
         res[prec] = run_gpu(gd, prec, ic, args.steps, args.warmup, args.repeats)
@@ -374,13 +569,27 @@ def main():
                 "avg_launch_us": round(mixed["kern_us"], 3),
                 "dominant_family_by_time": dom,
                 "family_ms_per_step": {k: round(v[1] / 60.0, 4) for k, v in prof.items()}}
-    cpu = None
-    if not args.no_cpu:
-        cv, cel = cpu_baseline(ic, args.cpu_steps)
-        cpu = {"value": round(cv, 6), "unit": "model-days/s", "cores": 1, "kind": "port",
-               "sample": f"first {args.cpu_steps} steps of the same 720x361 FP64 workload, "
-                         f"oracle/gd_oracle.c built -O3 -ffast-math (the reference's -Ofast), "
-                         f"single thread ({cel:.1f} s)"}
+    extra = {}
+    if args.extra:
+        eo, eroof, etuned = ensemble_workload(gd, args.ens_steps, 36)
+        extra["ensemble_64x360x181"] = {
+            "member_days_per_s": round(eo["mixed"], 3), "fp64_member_days_per_s": round(eo["fp64"], 3),
+            "mixed_over_fp64": round(eo["mixed"] / eo["fp64"], 3), "steps": args.ens_steps,
+            "autotuned": etuned, "roofline": {k: eroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
+        if cpu is not None:
+            extra["ensemble_64x360x181"]["over_cpu_processes"] = round(eo["mixed"] / cpu["ensemble"]["value"], 1)
+        wo, wroof = w1440_workload(gd, args.w1440_steps, 60)
+        extra["1440x721"] = {
+            "model_days_per_s": round(wo["mixed"], 4), "fp64_model_days_per_s": round(wo["fp64"], 4),
+            "mixed_over_fp64": round(wo["mixed"] / wo["fp64"], 3), "steps": args.w1440_steps,
+            "roofline": {k: wroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
+        extra["filter_implementations"] = filter_crossover(gd, 120, 240)
+        sv = serving_workload(gd, argparse.Namespace(serve_requests=args.serve_short), {})
+        extra["serving"] = {"model_days_per_s": sv["value"], "p50_s": sv["latency_s"]["p50"],
+                            "p99_s": sv["latency_s"]["p99"],
+                            "fp64_model_days_per_s": sv["config"]["fp64_model_days_per_s"],
+                            "fp64_p50_s": sv["latency_s"]["fp64_p50"], "fp64_p99_s": sv["latency_s"]["fp64_p99"],
+                            "requests": sv["config"]["requests"], "model_days": sv["config"]["model_days"]}
     lat = np.array(mixed["lat"]) if mixed["lat"] else np.array([np.nan])
     lat64 = np.array(res["fp64"]["lat"]) if res["fp64"]["lat"] else np.array([np.nan])
     out = {
@@ -407,7 +616,8 @@ def main():
                        "members": args.batch_members, "steps": args.batch_steps,
                        "aggregate_model_days_per_s": round(batched[0], 4),
                        "over_fp64_single_run": round(batched[0] / val["fp64"], 3)},
-                   "parallelism": "single GPU, single run"},
+                   "parallelism": "single GPU, single run",
+                   "other_workloads": extra or None},
         "latency_s": {"runs": int(len(mixed["lat"])), "p50": float(np.percentile(lat, 50)),
                       "p99": float(np.percentile(lat, 99)),
                       "fp64_p50": float(np.percentile(lat64, 50)),

@@ -21,7 +21,7 @@ GD_OK, GD_ERR_INVALID_ARG, GD_ERR_HIP, GD_ERR_NAN, GD_ERR_NOT_SUPPORTED = range(
 SPLIT = {"none": 0, "csp1": 1, "csp2": 2, "isp": 3}
 ADV = {"center_diff": 0, "upwind": 1, "weno": 2}
 PASS = {"all": 0, "fast": 1, "slow": 2}
-PRECISION = {"fp64": 0, "mixed": 1, "mixed_bf16": 2}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_bf16": 2, "mixed_mfma": 3}
 
 
 class DycoreError(RuntimeError):
@@ -94,6 +94,7 @@ EXPORTS = (
     "gd_restart_write", "gd_restart_read", "gd_io_last_error", "gd_ic_steady_geostrophic",
     "gd_ic_jet_zonal_flow", "gd_ic_shallow_water_waves", "gd_serve", "gd_serve_last_error",
     "gd_set_active_members", "gd_copy_member", "gd_fused_phases", "gd_filter_rows_noalloc",
+    "gd_autotune", "gd_server_create", "gd_server_submit", "gd_server_poll", "gd_server_destroy",
 )
 
 _lib = None
@@ -126,6 +127,7 @@ def lib():
     L.gd_filter_rows.argtypes = [vp, C.c_int32, dp, i64]
     L.gd_filter_table.argtypes = [vp, ip, ip, ip, ip]
     L.gd_filter_rows_noalloc.argtypes = [vp, C.c_int32, C.c_void_p, i64, vp]
+    L.gd_autotune.argtypes = [vp, C.c_int32, C.c_char_p, ip, ip, dp]
     L.gd_ic_balanced.argtypes = [C.c_int32, C.c_int32, C.POINTER(ICParams), dp, dp, dp, dp, i64]
     L.gd_ic_rossby_haurwitz.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double,
                                         C.c_double, dp, dp, dp, dp, i64]
@@ -154,6 +156,10 @@ def lib():
                            C.POINTER(ServeResult), dp]
     L.gd_serve_last_error.restype = C.c_char_p
     L.gd_serve_last_error.argtypes = []
+    L.gd_server_create.argtypes = [C.POINTER(ServeConfig), C.POINTER(vp)]
+    L.gd_server_submit.argtypes = [vp, C.POINTER(ServeRequest), C.POINTER(C.c_int64)]
+    L.gd_server_poll.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(ServeResult), ip]
+    L.gd_server_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -301,6 +307,72 @@ def read_restart(path, num_lon, num_lat):
 # serving: continuous batching of independent runs (gd_serve)
 # ---------------------------------------------------------------------------
 
+def _serve_config(precision, split_scheme, subcycles, qcon_modified, uv_adv_scheme, use_diffusion,
+                  diffusion_order, use_zonal_tend_filter, filter_lat0, hbm_budget_bytes, max_slots,
+                  chunk_steps, device):
+    cut = np.zeros(1, np.int32)
+    base = _Config(8, 5, 1.0, SPLIT[split_scheme], int(subcycles), int(bool(qcon_modified)),
+                   ADV[uv_adv_scheme], 0.0, 0.5, int(bool(use_diffusion)), int(diffusion_order), 0.0,
+                   int(bool(use_zonal_tend_filter)), 0, cut.ctypes.data_as(C.POINTER(C.c_int32)),
+                   PRECISION[precision], int(device))
+    cfg = ServeConfig(C.cast(C.pointer(base), C.c_void_p), float(filter_lat0), int(hbm_budget_bytes),
+                      int(max_slots), int(chunk_steps))
+    return cfg, (cut, base)
+
+
+class Server:
+    """Online continuous batching (gd_server_*): submit runs while others are
+    on the GPU, poll for their results.  Options as for serve()."""
+
+    def __init__(self, precision="mixed", split_scheme="csp2", subcycles=6, qcon_modified=True,
+                 uv_adv_scheme="center_diff", use_diffusion=True, diffusion_order=2,
+                 use_zonal_tend_filter=True, filter_lat0=75.0, hbm_budget_bytes=64 << 30, max_slots=64,
+                 chunk_steps=60, device=0):
+        self._cfg, self._keep = _serve_config(precision, split_scheme, subcycles, qcon_modified,
+                                              uv_adv_scheme, use_diffusion, diffusion_order,
+                                              use_zonal_tend_filter, filter_lat0, hbm_budget_bytes,
+                                              max_slots, chunk_steps, device)
+        self.s = C.c_void_p()
+        code = lib().gd_server_create(C.byref(self._cfg), C.byref(self.s))
+        if code != GD_OK:
+            raise DycoreError(code, lib().gd_serve_last_error().decode(errors="replace"))
+
+    def submit(self, num_lon, num_lat, time_step_size, nsteps, state, diffusion_coef=0.0):
+        """Submit one run (its state is copied); returns its id."""
+        N, M = int(num_lon), int(num_lat)
+        u, v, g, hs = (_dense(a, r, N) for a, r in zip(state, (M, M - 1, M, M)))
+        q = ServeRequest(N, M, float(time_step_size), float(diffusion_coef), int(nsteps), 0.0,
+                         _dp(u), _dp(v), _dp(g), _dp(hs))
+        rid = C.c_int64(-1)
+        code = lib().gd_server_submit(self.s, C.byref(q), C.byref(rid))
+        if code != GD_OK:
+            raise DycoreError(code, lib().gd_serve_last_error().decode(errors="replace"))
+        return rid.value
+
+    def poll(self, rid, wait=False):
+        """The run's result dict when it is done, else None."""
+        r = ServeResult()
+        done = C.c_int32(0)
+        code = lib().gd_server_poll(self.s, int(rid), int(bool(wait)), C.byref(r), C.byref(done))
+        if code != GD_OK:
+            raise DycoreError(code, lib().gd_serve_last_error().decode(errors="replace"))
+        return {f: getattr(r, f) for f, _ in ServeResult._fields_} if done.value else None
+
+    def close(self):
+        """Wait for every submitted run, then release the pools."""
+        if getattr(self, "s", None):
+            code = lib().gd_server_destroy(self.s)
+            self.s = None
+            if code != GD_OK:
+                raise DycoreError(code, lib().gd_serve_last_error().decode(errors="replace"))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def serve(requests, precision="mixed", split_scheme="csp2", subcycles=6, qcon_modified=True,
           uv_adv_scheme="center_diff", use_diffusion=True, diffusion_order=2,
           use_zonal_tend_filter=True, filter_lat0=75.0, hbm_budget_bytes=64 << 30, max_slots=64,
@@ -319,13 +391,9 @@ def serve(requests, precision="mixed", split_scheme="csp2", subcycles=6, qcon_mo
         keep += [u, v, g, hs]
         reqs[k] = ServeRequest(N, M, float(q["time_step_size"]), float(q.get("diffusion_coef", 0.0)),
                                int(q["nsteps"]), float(q["arrival_s"]), _dp(u), _dp(v), _dp(g), _dp(hs))
-    cut = np.zeros(1, np.int32)
-    base = _Config(8, 5, 1.0, SPLIT[split_scheme], int(subcycles), int(bool(qcon_modified)),
-                   ADV[uv_adv_scheme], 0.0, 0.5, int(bool(use_diffusion)), int(diffusion_order), 0.0,
-                   int(bool(use_zonal_tend_filter)), 0, cut.ctypes.data_as(C.POINTER(C.c_int32)),
-                   PRECISION[precision], int(device))
-    cfg = ServeConfig(C.cast(C.pointer(base), C.c_void_p), float(filter_lat0), int(hbm_budget_bytes),
-                      int(max_slots), int(chunk_steps))
+    cfg, keep_cfg = _serve_config(precision, split_scheme, subcycles, qcon_modified, uv_adv_scheme,
+                                  use_diffusion, diffusion_order, use_zonal_tend_filter, filter_lat0,
+                                  hbm_budget_bytes, max_slots, chunk_steps, device)
     res = (ServeResult * max(1, len(requests)))()
     wall = C.c_double(0.0)
     code = L.gd_serve(C.byref(cfg), reqs, len(requests), res, C.byref(wall))
@@ -479,14 +547,26 @@ class Dycore:
         """True when each tendency phase runs as one fused launch."""
         return bool(lib().gd_fused_phases(self.h))
 
+    def autotune(self, steps=40, cache=None):
+        """Pick the fastest launch shapes for the current active-member count
+        (gd_autotune; results stay bit for bit those of the defaults).
+        Returns {'fused', 'upd_wpm', 'ms_per_step', 'cache'}."""
+        f, w, ms = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
+        _check(lib().gd_autotune(self.h, int(steps), None if cache is None else str(cache).encode(),
+                                 C.byref(f), C.byref(w), C.byref(ms)), self.h)
+        return {"fused": bool(f.value), "upd_wpm": int(w.value), "ms_per_step": float(ms.value),
+                "cache": cache}
+
     def device_bytes(self):
         return int(lib().gd_device_bytes(self.h))
 
     def time_kernel(self, kernel, reps=200):
         """Microseconds per launch of one kernel replayed back to back:
-        kernel 'tend' (fast pass, second predictor), 'filter', 'update' or
-        'phase' (that phase as one fused launch; fused handles only)."""
+        kernel 'tend' (fast pass, second predictor), 'filter', 'update',
+        'phase' (that phase as one fused launch; fused handles only) or
+        'filter_mm' (the matrix-core projection of the momentum rows; the
+        mixed_mfma and mixed_bf16 modes only)."""
         us = C.c_double(0.0)
-        kid = {"tend": 0, "filter": 1, "update": 2, "phase": 3}[kernel]
+        kid = {"tend": 0, "filter": 1, "update": 2, "phase": 3, "filter_mm": 4}[kernel]
         _check(lib().gd_time_kernel(self.h, kid, int(reps), C.byref(us)), self.h)
         return us.value

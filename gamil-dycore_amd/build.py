@@ -24,8 +24,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-value",
 ]
-HIP_SOURCES = ["gd_kernels.hip", "gd_filter.hip", "gd_phase.hip", "gd_api.hip", "gd_host.cpp", "gd_io.cpp",
-               "gd_serve.cpp"]
+HIP_SOURCES = ["gd_kernels.hip", "gd_filter.hip", "gd_filter_mm.hip", "gd_phase.hip", "gd_api.hip", "gd_host.cpp",
+               "gd_io.cpp", "gd_serve.cpp"]
 
 
 def _run(cmd, cwd=None):

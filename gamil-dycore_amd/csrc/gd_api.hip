@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -46,18 +47,21 @@ struct gd_handle {
   std::vector<int32_t> cutoff;
   Geometry g;
   int nmem = 1, prec = 0, N = 0, M = 0, P = 0;
+  int filt_mm = 0;               // momentum filter rows on MFMA: 0 no (FFT), 1 FP32, 2 BF16 inputs
   int nact = 1;                  // members 0..nact-1 are stepped
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   int64_t bytes = 0;
   char *arena = nullptr;         // all members, mstride bytes apart
-  size_t mstride = 0, fs = 0, cs = 0, off_part = 0, off_dpart = 0, off_sx = 0, off_beta = 0;
+  size_t mstride = 0, fs = 0, cs = 0, off_part = 0, off_dpart = 0, off_sx = 0, off_beta = 0, off_sy = 0;
   RowCoef *dcoef = nullptr;
   std::vector<StageSlot> slots;
   StageSlot *dslots = nullptr;
   std::vector<FiltRow> frows;    // filter workgroups: (member, latitude) with staged rows
   FiltRow *dfrows = nullptr;
+  std::vector<int32_t> mslots;   // staging slots of the momentum rows (MFMA filter)
+  int32_t *dmslots = nullptr;
   double *dtwid = nullptr;       // [N/2+1][2] cos, sin of 2 pi k/N
   double *dghost = nullptr;      // [2][P]: 1.0 row, 0.0 row (stencil rows beyond the grid)
   int fft_L = 1, fft_half = 1;
@@ -81,6 +85,9 @@ struct gd_handle {
   // their filter workgroups (slot = position in the scratch array)
   double *scratch = nullptr;
   std::string scratch_err;
+  // autotuned launch shapes per active-member count (gd_autotune): fused (1)
+  // or separate (0) phase launches, k_update workgroups per member
+  std::map<int, int> tuned_fused, tuned_wpm;
   std::vector<int> prow[2];
   FiltRow *dprow[2] = {nullptr, nullptr};
 };
@@ -124,6 +131,8 @@ Args base_args(const gd_handle *h) {
   a.off_dpart = h->off_dpart;
   a.off_sx = h->off_sx;
   a.off_beta = h->off_beta;
+  a.off_sy = h->off_sy;
+  a.mslots = h->dmslots;
   a.diag = h->ddiag;
   a.coef = h->dcoef;
   a.slots = h->dslots;
@@ -149,6 +158,10 @@ Args base_args(const gd_handle *h) {
   // share about 512 workgroups
   a.nmem_launch = h->nact;
   a.upd_wpm = std::max(1, std::min((h->M + 1) / 2, 512 / h->nact));
+  {
+    auto it = h->tuned_wpm.find(h->nact);
+    if (it != h->tuned_wpm.end()) a.upd_wpm = it->second;
+  }
   a.upwind = h->cfg.uv_adv_scheme == GD_ADV_UPWIND;
   a.beta_lon = h->cfg.uv_adv_upwind_lon_beta;
   a.beta_lat = h->cfg.uv_adv_upwind_lat_beta;
@@ -178,6 +191,15 @@ hipError_t run(gd_handle *h, const char *family, F &&f) {
 
 hipError_t enqueue_filter(gd_handle *h, Args a) {
   if (h->frows.empty()) return hipSuccess;
+  if (h->filt_mm && !h->mslots.empty()) {
+    // the momentum rows of every active member projected on the matrix
+    // cores, one workgroup per (slot, 16 members); k_filter then finishes them
+    Args m = a;
+    m.ntiles = (int)h->mslots.size() * ((h->nact + 15) / 16);
+    hipError_t e = run(h, "filter_mm", [&] { return launch_filter_mm(m, h->filt_mm == 2, h->stream); });
+    if (e != hipSuccess) return e;
+    a.mm = 1;
+  }
   // member-major: the first nact members' rows
   a.ntiles = (int)(h->frows.size() / h->nmem) * h->nact;
   return run(h, "filter", [&] { return launch_filter(a, h->prec, h->stream); });
@@ -208,6 +230,10 @@ constexpr int64_t kFusedMaxTiles = 24576;
 
 bool use_fused(const gd_handle *h) {
   if (!h->fused_ok || h->fused_mode == 0) return false;
+  if (h->fused_mode == -1) {
+    auto it = h->tuned_fused.find(h->nact);
+    if (it != h->tuned_fused.end()) return it->second != 0;
+  }
   const int64_t tiles = (int64_t)h->nact * h->M * ((h->N + kSeg - 1) / kSeg);
   return h->fused_mode == 1 || tiles <= kFusedMaxTiles;
 }
@@ -505,7 +531,7 @@ std::string validate(const gd_config &c, int nmem) {
   if ((c.split_scheme == GD_SPLIT_CSP2 || c.split_scheme == GD_SPLIT_ISP) && c.subcycles < 1)
     return "subcycles must be >= 1";
   if (c.uv_adv_scheme < 0 || c.uv_adv_scheme > 2) return "unknown uv_adv_scheme";
-  if (c.precision < 0 || c.precision > 2) return "unknown precision";
+  if (c.precision < 0 || c.precision > 3) return "unknown precision";
   return "";
 }
 
@@ -523,6 +549,47 @@ hipError_t h2d_rows(gd_handle *h, double *dst, const double *src, int rows, int6
 hipError_t d2h_rows(gd_handle *h, double *dst, const double *src, int rows, int64_t ld) {
   return hipMemcpy2DAsync(dst, (size_t)ld * 8, src, (size_t)h->P * 8, (size_t)h->N * 8, rows,
                           hipMemcpyDeviceToHost, h->stream);
+}
+
+// Graphs captured with other launch shapes are dropped when the shapes change.
+void drop_graphs(gd_handle *h) {
+  hipStreamSynchronize(h->stream);
+  for (auto &kv : h->graphs) hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
+  h->graph_failed = false;
+}
+
+// Autotuner cache key: everything that decides the launch shapes' timing.
+std::string tune_key(const gd_handle *h, const char *arch) {
+  const gd_config &c = h->cfg;
+  char b[320];
+  std::snprintf(b, sizeof b,
+                "arch=%s N=%d M=%d members=%d active=%d prec=%d split=%d sub=%d adv=%d diff=%d filt=%zu",
+                arch, h->N, h->M, h->nmem, h->nact, h->prec, c.split_scheme, c.subcycles, c.uv_adv_scheme,
+                c.use_diffusion ? c.diffusion_order : 0, h->frows.size() / (size_t)h->nmem);
+  return b;
+}
+
+// Cache lines: "<key> | fused=F wpm=W ms=X"; the last line of a key wins.
+bool tune_lookup(const char *path, const std::string &key, int &fused, int &wpm, double &ms) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) return false;
+  char line[512];
+  bool found = false;
+  while (std::fgets(line, sizeof line, f)) {
+    const char *bar = std::strstr(line, " | ");
+    if (!bar || std::string(line, bar - line) != key) continue;
+    int fu, w;
+    double x;
+    if (std::sscanf(bar + 3, "fused=%d wpm=%d ms=%lf", &fu, &w, &x) == 3) {
+      fused = fu;
+      wpm = w;
+      ms = x;
+      found = true;
+    }
+  }
+  std::fclose(f);
+  return found;
 }
 
 int read_tend(gd_handle *h, const Member &mb, int comp, double *out, int rows, int64_t ld) {
@@ -547,8 +614,6 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   *out = nullptr;
   std::string v = validate(*cfg, n_members);
   if (!v.empty()) return fail(nullptr, GD_ERR_INVALID_ARG, v);
-  if (cfg->precision == GD_PREC_MIXED_BF16)
-    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "precision 'mixed_bf16' is not provided");
   if (cfg->uv_adv_scheme == GD_ADV_WENO)
     return fail(nullptr, GD_ERR_NOT_SUPPORTED, "uv_adv_scheme 'weno' is not provided on the device path");
   if (cfg->use_diffusion && cfg->diffusion_order < 2)
@@ -560,6 +625,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->nmem = n_members;
   h->nact = n_members;
   h->prec = cfg->precision == GD_PREC_FP64 ? 0 : 1;
+  h->filt_mm = cfg->precision == GD_PREC_MIXED_MFMA ? 1 : (cfg->precision == GD_PREC_MIXED_BF16 ? 2 : 0);
   h->N = cfg->num_lon;
   h->M = cfg->num_lat;
   h->P = ((h->N + 31) / 32) * 32;
@@ -635,8 +701,13 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
     h->nfar_rows = (int)far.size();
     const char *env = std::getenv("GD_FUSED");
     h->fused_mode = env && env[0] == '0' ? 0 : (env && env[0] == '1' ? 1 : -1);
-    h->fused_ok = fits && !h->frows.empty() && h->N <= 4 * kBlock &&
+    // (the MFMA filter runs as its own launch between k_tend and k_filter)
+    h->fused_ok = fits && !h->frows.empty() && h->N <= 4 * kBlock && !h->filt_mm &&
                   phase_lds_bytes(h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
+  }
+  if (h->filt_mm && (h->N < 64 || h->N > 1440 || filter_mm_lds_bytes(h->N, h->filt_mm == 2) > prop.sharedMemPerBlock)) {
+    delete h;
+    return fail(nullptr, GD_ERR_NOT_SUPPORTED, "the MFMA polar filter takes 64 <= num_lon <= 1440");
   }
   configure_kernels((int)prop.sharedMemPerBlock);
   hipGetLastError();
@@ -656,7 +727,8 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   h->off_dpart = h->off_part + al((size_t)h->M * 3 * a_nseg * 2 * 8);
   h->off_sx = h->off_dpart + al((size_t)h->M * a_nseg * 4 * 8);
   h->off_beta = h->off_sx + al(std::max<size_t>(nslots, 1) * h->P * 8);
-  h->mstride = h->off_beta + al(8);
+  h->off_sy = h->off_beta + al(8);
+  h->mstride = h->off_sy + (h->filt_mm ? al(std::max<size_t>(nslots, 1) * h->P * 8) : 0);
   h->arena = (char *)dalloc(h, h->mstride * h->nmem, e);
   if (!h->arena) { gd_destroy(h); return fail(nullptr, GD_ERR_HIP, "device allocation failed"); }
   e = hipMemsetAsync(h->arena, 0, h->mstride * h->nmem, h->stream);
@@ -680,8 +752,12 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   }
   for (int k = 0; k < 2; ++k)
     h->dprow[k] = (FiltRow *)dalloc(h, sizeof(FiltRow) * std::max<size_t>(prow_tab[k].size(), 1), e);
+  h->mslots.clear();
+  for (size_t k = 0; k < h->slots.size(); ++k)
+    if (h->slots[k].comp != CDGD) h->mslots.push_back((int32_t)k);
+  h->dmslots = (int32_t *)dalloc(h, sizeof(int32_t) * std::max<size_t>(h->mslots.size(), 1), e);
   if (!h->ddiag || !h->dcoef || !h->dslots || !h->dfrows || !h->dtwid ||
-      !h->dstep || !h->dghost || !h->dprow[0] || !h->dprow[1]) {
+      !h->dstep || !h->dghost || !h->dprow[0] || !h->dprow[1] || !h->dmslots) {
     gd_destroy(h);
     return fail(nullptr, GD_ERR_HIP, "device allocation failed");
   }
@@ -699,6 +775,9 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   if (!h->frows.empty())
     hipMemcpyAsync(h->dfrows, h->frows.data(), sizeof(FiltRow) * h->frows.size(), hipMemcpyHostToDevice, h->stream);
   hipMemcpyAsync(h->dtwid, tw.data(), tw.size() * 8, hipMemcpyHostToDevice, h->stream);
+  if (!h->mslots.empty())
+    hipMemcpyAsync(h->dmslots, h->mslots.data(), sizeof(int32_t) * h->mslots.size(), hipMemcpyHostToDevice,
+                   h->stream);
   for (int k = 0; k < 2; ++k)
     if (!prow_tab[k].empty())
       hipMemcpyAsync(h->dprow[k], prow_tab[k].data(), sizeof(FiltRow) * prow_tab[k].size(), hipMemcpyHostToDevice,
@@ -1019,9 +1098,97 @@ int gd_copy_member(gd_handle *h, int32_t dst, int32_t src) {
 
 int gd_fused_phases(const gd_handle *h) { return h && use_fused(h) ? 1 : 0; }
 
+int gd_autotune(gd_handle *h, int32_t steps, const char *cache_path, int32_t *fused_out, int32_t *wpm_out,
+                double *ms_out) {
+  if (!h) return GD_ERR_INVALID_ARG;
+  if (steps < 1 || steps > h->diag_cap) return fail(h, GD_ERR_INVALID_ARG, "autotune steps must be in [1, 512]");
+  hipSetDevice(h->device);
+  hipDeviceProp_t prop;
+  GD_HIP(h, hipGetDeviceProperties(&prop, h->device));
+  const std::string key = tune_key(h, prop.gcnArchName);
+  const int nact = h->nact;
+  int bf = -1, bw = -1;
+  double bms = 0.0;
+  bool cached = cache_path && tune_lookup(cache_path, key, bf, bw, bms);
+  if (cached && (bw < 1 || bw > h->M || (bf && !h->fused_ok))) cached = false;
+  if (!cached) {
+    // fused or separate phase launches (both give the same bits; a forced
+    // GD_FUSED keeps its form) ...
+    std::vector<int> fopts;
+    if (h->fused_mode == 0 || !h->fused_ok)
+      fopts = {0};
+    else if (h->fused_mode == 1)
+      fopts = {1};
+    else
+      fopts = {0, 1};
+    // ... and k_update rows per workgroup (each workgroup forms its member's
+    // beta from all of the member's partials in one order: the same bits)
+    std::vector<int> wopts;
+    for (int rpw : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256}) {
+      const int w = (h->M + rpw - 1) / rpw;
+      if (w >= 1 && (int64_t)w * nact <= 65536 && std::find(wopts.begin(), wopts.end(), w) == wopts.end())
+        wopts.push_back(w);
+    }
+    const int wdef = base_args(h).upd_wpm;
+    if (std::find(wopts.begin(), wopts.end(), wdef) == wopts.end()) wopts.push_back(wdef);
+    // every candidate steps from the same saved state, which is put back at
+    // the end (a set-up call: the copy is allocated here)
+    const size_t nb = (size_t)nact * h->mstride;
+    void *save = nullptr;
+    GD_HIP(h, hipMalloc(&save, nb));
+    auto restore = [&]() { return hipMemcpyAsync(h->arena, save, nb, hipMemcpyDeviceToDevice, h->stream); };
+    hipError_t e = hipMemcpyAsync(save, h->arena, nb, hipMemcpyDeviceToDevice, h->stream);
+    int rc = e == hipSuccess ? GD_OK : hip_fail(h, e, "autotune save");
+    for (int f : fopts) {
+      for (int w : wopts) {
+        if (rc) break;
+        h->tuned_fused[nact] = f;
+        h->tuned_wpm[nact] = w;
+        drop_graphs(h);
+        if ((e = restore()) != hipSuccess) { rc = hip_fail(h, e, "autotune restore"); break; }
+        if ((rc = gd_step(h, 3, nullptr, nullptr, nullptr, nullptr))) break;  // capture + warm
+        if ((e = restore()) != hipSuccess) { rc = hip_fail(h, e, "autotune restore"); break; }
+        if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) { rc = hip_fail(h, e, "autotune"); break; }
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((rc = gd_step(h, steps, nullptr, nullptr, nullptr, nullptr))) break;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        if (bw < 0 || ms < bms) {
+          bf = f;
+          bw = w;
+          bms = ms;
+        }
+      }
+    }
+    if ((e = restore()) != hipSuccess && !rc) rc = hip_fail(h, e, "autotune restore");
+    hipStreamSynchronize(h->stream);
+    hipFree(save);
+    if (rc) {
+      h->tuned_fused.erase(nact);
+      h->tuned_wpm.erase(nact);
+      drop_graphs(h);
+      return rc;
+    }
+  }
+  h->tuned_fused[nact] = bf;
+  h->tuned_wpm[nact] = bw;
+  drop_graphs(h);
+  if (!cached && cache_path) {
+    if (FILE *f = std::fopen(cache_path, "a")) {
+      std::fprintf(f, "%s | fused=%d wpm=%d ms=%.17g\n", key.c_str(), bf, bw, bms);
+      std::fclose(f);
+    }
+  }
+  if (fused_out) *fused_out = bf;
+  if (wpm_out) *wpm_out = bw;
+  if (ms_out) *ms_out = bms;
+  return GD_OK;
+}
+
 int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us) {
-  if (!h || reps < 1 || !us || kernel < GD_TIME_TEND || kernel > GD_TIME_PHASE) return GD_ERR_INVALID_ARG;
+  if (!h || reps < 1 || !us || kernel < GD_TIME_TEND || kernel > GD_TIME_FILTER_MM) return GD_ERR_INVALID_ARG;
   if (kernel == GD_TIME_FILTER && h->frows.empty()) return fail(h, GD_ERR_INVALID_ARG, "no filtered rows");
+  if (kernel == GD_TIME_FILTER_MM && (!h->filt_mm || h->mslots.empty()))
+    return fail(h, GD_ERR_NOT_SUPPORTED, "the handle does not filter on the matrix cores");
   if (kernel == GD_TIME_PHASE && !use_fused(h))
     return fail(h, GD_ERR_NOT_SUPPORTED, "the handle does not use fused phase launches");
   hipSetDevice(h->device);
@@ -1058,6 +1225,12 @@ int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us) {
       Args f = a;
       set_fused(h, f, h->nmem);
       return launch_phase(f, h->prec, pass, false, h->stream);
+    }
+    if (kernel == GD_TIME_FILTER_MM) {
+      Args f = a;
+      // the kernel's member blocks follow a.nmem (the active members)
+      f.ntiles = (int)h->mslots.size() * ((f.nmem + 15) / 16);
+      return launch_filter_mm(f, h->filt_mm == 2, h->stream);
     }
     return launch_update(a, h->prec, h->stream);
   };

@@ -11,7 +11,8 @@
 //        (dycore_mod.F90:636-645)
 //   P1,P2 - the two predictor states of predict_correct
 // Tendencies Tb (second predictor) and Tc (corrector), FP64 in every
-// precision mode, feed the beta inner products.
+// precision mode, feed the beta inner products.  In the MFMA filter modes the
+// projected momentum rows are staged at off_sy beside the rows at off_sx.
 #pragma once
 
 #include <cstdint>
@@ -62,7 +63,7 @@ struct RowCoef {
 //                                     [nslots][pitch], beta of the last call
 struct Member {
   char *b;
-  size_t fs, cs, off_part, off_dpart, off_sx, off_beta;
+  size_t fs, cs, off_part, off_dpart, off_sx, off_beta, off_sy;
   double *diag_out;  // [cap][3]: mass, energy, beta per step
   __host__ __device__ double *st(int slot, int f) const { return (double *)(b + (size_t)(slot * NFLD + f) * fs); }
   __host__ __device__ double *ghs() const { return (double *)(b + (size_t)(NSLOT * NFLD) * fs); }
@@ -78,6 +79,8 @@ struct Member {
   __host__ __device__ double *dpart() const { return (double *)(b + off_dpart); }
   __host__ __device__ double *sx() const { return (double *)(b + off_sx); }
   __host__ __device__ double *beta() const { return (double *)(b + off_beta); }
+  // projected momentum rows of the MFMA filter [nslots][pitch] (MFMA modes)
+  __host__ __device__ double *sy() const { return (double *)(b + off_sy); }
 };
 
 struct StageSlot {
@@ -101,11 +104,13 @@ struct FiltRow {
 
 struct Args {
   char *arena;                 // member m at arena + m * mstride
-  size_t mstride, fs, cs, off_part, off_dpart, off_sx, off_beta;
+  size_t mstride, fs, cs, off_part, off_dpart, off_sx, off_beta, off_sy;
   double *diag;                // [nmem][diag_cap][3]
   const RowCoef *coef;
   const StageSlot *slots;
   const FiltRow *frows;
+  const int32_t *mslots;       // k_filter_mm: the staging slots of the momentum rows
+  int mm;                      // k_filter: momentum rows come projected in sy (MFMA modes)
   const double *twid;          // [N/2+1][2]: cos, sin of 2 pi k / N
   const double *ghost;         // [2][pitch]: a row of 1.0 and a row of 0.0, read for rows beyond the grid
   double *sx_ext;              // k_filter projection (PH_P): staged rows at sx_ext + slot * pitch
@@ -154,6 +159,7 @@ __host__ __device__ inline Member member_at(const Args &a, int m) {
   mb.off_dpart = a.off_dpart;
   mb.off_sx = a.off_sx;
   mb.off_beta = a.off_beta;
+  mb.off_sy = a.off_sy;
   mb.diag_out = a.diag + (size_t)m * a.diag_cap * 3;
   return mb;
 }

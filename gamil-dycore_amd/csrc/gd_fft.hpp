@@ -377,7 +377,20 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
   // after filtering (dycore_mod.F90:211, 214), reduced together
   double yv[KP];
   double s12[2] = {0.0, 0.0};
-  if (!zero) {
+  // MFMA modes: a momentum row arrives projected (k_filter_mm) in the
+  // member's second staging area; only the rescale and epilogue remain
+  const bool proj = !XIN && a.mm && comp != CDGD;
+  if (proj) {
+    const double *yp = mb.sy() + (size_t)fr.slot * P;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int l = tid + k * kBlock;
+      yv[k] = l < N ? yp[l] : 0.0;
+      s12[0] += x[k] * w[k];
+      s12[1] += yv[k] * w[k];
+    }
+    if (rescale) block_sum_n<2>(s12, red);
+  } else if (!zero) {
     const Tw twt{tw, N};
     int cur = fft_any(b0, b1, 0, a, twt, false);
     if (gdrow) {

@@ -497,7 +497,9 @@ hipError_t launch_divvor(const Args &a, hipStream_t s) {
 
 // Raise the dynamic-LDS limit of the kernels that size LDS by num_lon.
 hipError_t configure_kernels(int max_lds) {
-  hipError_t e = configure_filter(max_lds);
+  hipError_t e = configure_filter_mm(max_lds);
+  if (e != hipSuccess) return e;
+  e = configure_filter(max_lds);
   return e != hipSuccess ? e : configure_phase(max_lds);
 }
 

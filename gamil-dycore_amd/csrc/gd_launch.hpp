@@ -44,6 +44,11 @@ hipError_t launch_phase(const Args &a, int prec, int pass, bool in_full, hipStre
 // (gd_phase.hip); N <= 4 * kBlock
 hipError_t launch_dphase(const Args &a, int prec, hipStream_t s);
 size_t phase_lds_bytes(int N, int L, int prec);
+// momentum rows of the filter as batched MFMA products (gd_filter_mm.hip):
+// one workgroup per (momentum slot in a.mslots, block of 16 members)
+hipError_t launch_filter_mm(const Args &a, bool bf16, hipStream_t s);
+hipError_t configure_filter_mm(int max_lds);
+size_t filter_mm_lds_bytes(int N, bool bf16);
 hipError_t configure_phase(int max_lds);
 
 }  // namespace gd

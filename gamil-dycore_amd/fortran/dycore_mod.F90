@@ -126,6 +126,8 @@ contains
       cfg%precision = 1
     case ('mixed_bf16')
       cfg%precision = 2
+    case ('mixed_mfma')
+      cfg%precision = 3
     case default
       call log_error('Unknown precision ' // trim(precision) // '!')
     end select

@@ -4,7 +4,8 @@
 ! Superset keys:
 !   zonal_tend_filter_lat0  > 0: build the polar filter table from the rule
 !                           c_k = floor(num_lon/2 cos(lat_k)/cos(lat0))
-!   precision               'fp64' | 'mixed' | 'mixed_bf16'
+!   precision               'fp64' | 'mixed' | 'mixed_mfma' | 'mixed_bf16' (the last two
+!                           filter the momentum rows on the matrix cores, FP32 or BF16 inputs)
 !   ensemble_members        number of device-resident runs of this config
 !   device                  HIP device ordinal
 !   log_every               print the per-step diagnostics line every n steps

@@ -49,7 +49,10 @@ enum {
   GD_PREC_MIXED = 1,       /* FP32 momentum stencils; FP64 state update, stored
                               tendencies, pole sums, mass flux divergence, polar
                               filter and integrals                               */
-  GD_PREC_MIXED_BF16 = 2   /* reserved: gd_create returns GD_ERR_NOT_SUPPORTED     */
+  GD_PREC_MIXED_BF16 = 2,  /* MIXED with the polar filter of the momentum rows as
+                              batched matrix-core products, BF16 inputs and FP32
+                              accumulation (64 <= num_lon <= 1440)               */
+  GD_PREC_MIXED_MFMA = 3   /* the same with FP32 inputs                           */
 };
 
 /* &dycore_params (src/params_mod.F90:13-98) plus the device knobs. */
@@ -329,13 +332,32 @@ typedef struct gd_serve_config {
   double filter_lat0;          /* per-class filter table from cos(lat)/cos(lat0)
                                   (gd_cutoff_from_lat0); 0: no filter rows       */
   int64_t hbm_budget_bytes;    /* device memory for resident runs               */
-  int32_t max_slots;           /* members per pool at most                       */
+  int32_t max_slots;           /* members (slots) per pool at most              */
   int32_t chunk_steps;         /* steps between admission checks                 */
 } gd_serve_config;
 
-/* Serves the n requests (arrival times relative to the call) and returns when
- * every run has finished; res[k] belongs to req[k].  wall_seconds (may be
- * NULL) receives the duration of the stream. */
+/* Online server: runs are submitted while others are on the GPU and polled
+ * for their results.  A run's class gets a pool (one handle, runs as
+ * members) when it has runs waiting and none live, with as many slots (up to
+ * max_slots) as the budget left by the live pools and the free device memory
+ * allow; a pool idle for 50 ms is released and its bytes go back to the
+ * budget.  Waiting runs are admitted into free slots between chunks of at
+ * most chunk_steps steps; a run whose mass or energy becomes NaN is evicted
+ * with status GD_ERR_NAN (the other runs are not affected).
+ * gd_server_submit copies the request's state (arrival_s is ignored: the run
+ * arrives at the call) and returns its id; gd_server_poll returns its result
+ * in *res when done (*done = 1; wait != 0 blocks until then);
+ * gd_server_destroy waits for every submitted run, then frees the pools. */
+typedef struct gd_server gd_server;
+int gd_server_create(const gd_serve_config *cfg, gd_server **out);
+int gd_server_submit(gd_server *s, const gd_serve_request *req, int64_t *id);
+int gd_server_poll(gd_server *s, int64_t id, int32_t wait, gd_serve_result *res, int32_t *done);
+int gd_server_destroy(gd_server *s);
+
+/* Serves the n requests (arrival times relative to the call) through a
+ * server, submitting each at its arrival time, and returns when every run
+ * has finished; res[k] belongs to req[k].  wall_seconds (may be NULL)
+ * receives the duration of the stream. */
 int gd_serve(const gd_serve_config *cfg, const gd_serve_request *req, int32_t n,
              gd_serve_result *res, double *wall_seconds);
 const char *gd_serve_last_error(void);
@@ -360,9 +382,11 @@ int64_t gd_device_bytes(const gd_handle *h);
  * pass, second predictor), GD_TIME_FILTER (the filter of that phase),
  * GD_TIME_UPDATE (beta + corrector), GD_TIME_PHASE (that phase as one fused
  * launch: tendency, filter rows and epilogue; GD_ERR_NOT_SUPPORTED when the
- * handle steps with separate tendency and filter launches).  No counterpart
- * in the reference: a measurement hook. */
-enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2, GD_TIME_PHASE = 3 };
+ * handle steps with separate tendency and filter launches), GD_TIME_FILTER_MM
+ * (the matrix-core projection of the momentum rows of every member; the
+ * mixed_mfma and mixed_bf16 modes only).  No counterpart in the reference: a
+ * measurement hook. */
+enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2, GD_TIME_PHASE = 3, GD_TIME_FILTER_MM = 4 };
 
 /* 1 when the handle's step, at its current active-member count, runs each
  * tendency phase as one fused launch, else 0.  Phases are fused for
@@ -371,6 +395,20 @@ enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2, GD_TIME_PHASE =
  * when the handle is created forces separate or fused launches.  No
  * counterpart in the reference. */
 int gd_fused_phases(const gd_handle *h);
+
+/* Tile autotuner (no counterpart in the reference).  For the current
+ * active-member count, times `steps` steps (1..512) of every launch shape
+ * that cannot change a result -- phases fused or as separate tendency and
+ * filter launches, and the workgroups per member of the corrector update --
+ * from the members' current state, puts that state back, and keeps the
+ * fastest shape for later gd_step calls at this active-member count (the
+ * results are bit for bit those of the default shapes).  cache_path (may be
+ * NULL) is a text file of earlier results keyed by the device architecture,
+ * grid, member counts, precision and schemes: a cached entry is applied
+ * without timing, a new result is appended.  Outputs (may be NULL): the
+ * chosen fused flag, workgroups per member and milliseconds per step. */
+int gd_autotune(gd_handle *h, int32_t steps, const char *cache_path, int32_t *fused,
+                int32_t *upd_wpm, double *ms_per_step);
 int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us_per_launch);
 
 #ifdef __cplusplus

@@ -120,13 +120,9 @@ def test_serve_rejects_bad_requests(gd):
 def test_create_rejects_bad_configs_before_any_device_call(gd):
     """gd_create validates the configuration before it touches a device:
     isp needs subcycles >= 1 like csp2 (isp_splitting divides dt and the
-    accumulator scale by it, dycore_mod.F90:689-752), and the BF16 precision
-    mode is refused instead of running as another mode."""
+    accumulator scale by it, dycore_mod.F90:689-752)."""
     with pytest.raises(gd.DycoreError, match="subcycles") as e:
         gd.Dycore(72, 37, 600.0, split_scheme="isp", subcycles=0)
     assert e.value.code == gd.GD_ERR_INVALID_ARG
     with pytest.raises(gd.DycoreError, match="subcycles"):
         gd.Dycore(72, 37, 600.0, split_scheme="csp2", subcycles=-1)
-    with pytest.raises(gd.DycoreError, match="mixed_bf16") as e:
-        gd.Dycore(72, 37, 600.0, precision="mixed_bf16")
-    assert e.value.code == gd.GD_ERR_NOT_SUPPORTED

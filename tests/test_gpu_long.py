@@ -9,9 +9,11 @@ seeded states through the C ABI in FP64 and in mixed precision (the ensemble
 as one 64-member handle).
 
 Tolerances: FP64 agrees with the oracle to round-off growth (height l2 1e-12,
-winds 1e-10, mass and energy 1e-12 relative); mixed meets the stated target,
-height l2 <= 1e-5 relative with mass drift <= 1e-12 of the initial mass, and
-energy within 1e-8 of the oracle's.  Full-size properties with neither
+winds 1e-10, mass and energy 1e-12 relative); mixed, and mixed with the FP32
+matrix-core momentum filter, meet the stated target, height l2 <= 1e-5
+relative with mass drift <= 1e-12 of the initial mass, and energy within
+1e-8 of the oracle's; with the BF16 matrix-core filter the height l2 bound is
+1e-4 (mass drift still <= 1e-12: the mass rows keep the FP64 filter).  Full-size properties with neither
 filter nor diffusion: mass conserved to round-off, and with the
 energy-conserving beta the quadratic energy too.
 """
@@ -67,9 +69,14 @@ def _device_run(gd, p, prec):
     return out[p.get("member", 0)]
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+PRECS = ["fp64", "mixed", "mixed_mfma", "mixed_bf16"]
+
+
+@pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("name", CASES)
 def test_config_vs_oracle(gd, name, prec):
+    if prec.startswith("mixed_") and name == "360x181_1day":
+        pytest.skip("the matrix-core filter modes run on the 1440x721, 720x361 and ensemble cases")
     p, f = _fixture(name)
     s = p["stride"]
     (u, v, h), m0, mass, energy = _device_run(gd, p, prec)
@@ -87,6 +94,9 @@ def test_config_vs_oracle(gd, name, prec):
     if prec == "fp64":
         assert eh <= 1e-12 and eu <= 1e-10 and ev <= 1e-10
         assert om <= 1e-12 and de <= 1e-12
+    elif prec == "mixed_bf16":
+        # BF16 split operands in the momentum filter: stated, looser
+        assert eh <= 1e-4 and eu <= 1e-3 and ev <= 1e-3
     else:
         assert eh <= 1e-5 and linf_h <= 1e-4
         assert eu <= 1e-4 and ev <= 1e-4
