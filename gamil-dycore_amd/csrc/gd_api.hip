@@ -62,6 +62,7 @@ struct gd_handle {
   FiltRow *dfrows = nullptr;
   std::vector<int32_t> mslots;   // staging slots of the momentum rows (MFMA filter)
   int32_t *dmslots = nullptr;
+  void *dmm_phi = nullptr;       // basis matrices of the MFMA filter
   double *dtwid = nullptr;       // [N/2+1][2] cos, sin of 2 pi k/N
   double *dghost = nullptr;      // [2][P]: 1.0 row, 0.0 row (stencil rows beyond the grid)
   int fft_L = 1, fft_half = 1;
@@ -133,6 +134,7 @@ Args base_args(const gd_handle *h) {
   a.off_beta = h->off_beta;
   a.off_sy = h->off_sy;
   a.mslots = h->dmslots;
+  a.mm_phi = h->dmm_phi;
   a.diag = h->ddiag;
   a.coef = h->dcoef;
   a.slots = h->dslots;
@@ -227,6 +229,10 @@ void set_fused(const gd_handle *h, Args &a, int nm) {
 // 360x181 (23,168) and 128 of 180x91 (11,648); even at 32 runs of 720x361
 // (34,656).
 constexpr int64_t kFusedMaxTiles = 24576;
+// Rows past 1024 longitudes (eight per thread in the filter tiles; three
+// tiles per CU): a 1440x721 run (4,326 row tiles) measured 4% faster fused;
+// larger launches are left to the autotuner.
+constexpr int64_t kFusedMaxTilesWide = 6144;
 
 bool use_fused(const gd_handle *h) {
   if (!h->fused_ok || h->fused_mode == 0) return false;
@@ -235,7 +241,7 @@ bool use_fused(const gd_handle *h) {
     if (it != h->tuned_fused.end()) return it->second != 0;
   }
   const int64_t tiles = (int64_t)h->nact * h->M * ((h->N + kSeg - 1) / kSeg);
-  return h->fused_mode == 1 || tiles <= kFusedMaxTiles;
+  return h->fused_mode == 1 || tiles <= (h->N <= 4 * kBlock ? kFusedMaxTiles : kFusedMaxTilesWide);
 }
 
 // One tendency phase (a.phase) of the active members with its filter rows:
@@ -702,7 +708,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
     const char *env = std::getenv("GD_FUSED");
     h->fused_mode = env && env[0] == '0' ? 0 : (env && env[0] == '1' ? 1 : -1);
     // (the MFMA filter runs as its own launch between k_tend and k_filter)
-    h->fused_ok = fits && !h->frows.empty() && h->N <= 4 * kBlock && !h->filt_mm &&
+    h->fused_ok = fits && !h->frows.empty() && h->N <= 8 * kBlock && !h->filt_mm &&
                   phase_lds_bytes(h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
   }
   if (h->filt_mm && (h->N < 64 || h->N > 1440 || filter_mm_lds_bytes(h->N, h->filt_mm == 2) > prop.sharedMemPerBlock)) {
@@ -756,6 +762,15 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   for (size_t k = 0; k < h->slots.size(); ++k)
     if (h->slots[k].comp != CDGD) h->mslots.push_back((int32_t)k);
   h->dmslots = (int32_t *)dalloc(h, sizeof(int32_t) * std::max<size_t>(h->mslots.size(), 1), e);
+  std::vector<unsigned char> mm_basis;
+  if (h->filt_mm) {
+    mm_basis_fill(h->N, h->filt_mm == 2, mm_basis);
+    h->dmm_phi = dalloc(h, mm_basis.size(), e);
+    if (!h->dmm_phi) {
+      gd_destroy(h);
+      return fail(nullptr, GD_ERR_HIP, "device allocation failed");
+    }
+  }
   if (!h->ddiag || !h->dcoef || !h->dslots || !h->dfrows || !h->dtwid ||
       !h->dstep || !h->dghost || !h->dprow[0] || !h->dprow[1] || !h->dmslots) {
     gd_destroy(h);
@@ -778,6 +793,7 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
   if (!h->mslots.empty())
     hipMemcpyAsync(h->dmslots, h->mslots.data(), sizeof(int32_t) * h->mslots.size(), hipMemcpyHostToDevice,
                    h->stream);
+  if (h->dmm_phi) hipMemcpyAsync(h->dmm_phi, mm_basis.data(), mm_basis.size(), hipMemcpyHostToDevice, h->stream);
   for (int k = 0; k < 2; ++k)
     if (!prow_tab[k].empty())
       hipMemcpyAsync(h->dprow[k], prow_tab[k].data(), sizeof(FiltRow) * prow_tab[k].size(), hipMemcpyHostToDevice,
@@ -1109,6 +1125,7 @@ int gd_autotune(gd_handle *h, int32_t steps, const char *cache_path, int32_t *fu
   const int nact = h->nact;
   int bf = -1, bw = -1;
   double bms = 0.0;
+  if (cache_path && !cache_path[0]) cache_path = nullptr;  // "" : no cache
   bool cached = cache_path && tune_lookup(cache_path, key, bf, bw, bms);
   if (cached && (bw < 1 || bw > h->M || (bf && !h->fused_ok))) cached = false;
   if (!cached) {

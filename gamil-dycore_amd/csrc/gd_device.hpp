@@ -110,6 +110,7 @@ struct Args {
   const StageSlot *slots;
   const FiltRow *frows;
   const int32_t *mslots;       // k_filter_mm: the staging slots of the momentum rows
+  const void *mm_phi;          // k_filter_mm: the basis matrices (mm_basis_fill)
   int mm;                      // k_filter: momentum rows come projected in sy (MFMA modes)
   const double *twid;          // [N/2+1][2]: cos, sin of 2 pi k / N
   const double *ghost;         // [2][pitch]: a row of 1.0 and a row of 0.0, read for rows beyond the grid

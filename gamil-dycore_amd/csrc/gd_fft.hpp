@@ -240,8 +240,14 @@ __device__ __forceinline__ int fft_fixed(double2 *b0, double2 *b1, int cur, Tw t
 }
 
 // The plans of the even row lengths 720, 360 and 180 (fft_plan of their
-// half lengths) run fixed; others run through fft_run.
+// half lengths) run fixed, and 1440 (half length 720 = 8 2 3 3 5) where the
+// row takes eight or more points per thread (KP); others run through
+// fft_run.
+template <int KP>
 __device__ __forceinline__ int fft_any(double2 *b0, double2 *b1, int cur, const Args &a, Tw tw, bool inv) {
+  if constexpr (KP >= 8) {
+    if (a.fft_half && a.fft_L == 720) return fft_fixed<720, 2, 1, 8, 2, 3, 3, 5>(b0, b1, cur, tw, inv);
+  }
   if (a.fft_half && a.fft_L == 360) return fft_fixed<360, 2, 1, 8, 3, 3, 5>(b0, b1, cur, tw, inv);
   if (a.fft_half && a.fft_L == 180) return fft_fixed<180, 2, 1, 4, 3, 3, 5>(b0, b1, cur, tw, inv);
   if (a.fft_half && a.fft_L == 90) return fft_fixed<90, 2, 1, 2, 3, 3, 5>(b0, b1, cur, tw, inv);
@@ -392,7 +398,7 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
     if (rescale) block_sum_n<2>(s12, red);
   } else if (!zero) {
     const Tw twt{tw, N};
-    int cur = fft_any(b0, b1, 0, a, twt, false);
+    int cur = fft_any<KP>(b0, b1, 0, a, twt, false);
     if (gdrow) {
 #pragma unroll
       for (int k = 0; k < KP; ++k) w[k] = w[k] + wg[k];
@@ -402,7 +408,7 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
     else
       cplx_mid(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, sc);
     __syncthreads();
-    cur = fft_any(b0, b1, cur ^ 1, a, twt, true);
+    cur = fft_any<KP>(b0, b1, cur ^ 1, a, twt, true);
     const double *y = (const double *)(cur ? b1 : b0);
 #pragma unroll
     for (int k = 0; k < KP; ++k) {

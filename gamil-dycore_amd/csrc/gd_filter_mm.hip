@@ -18,11 +18,18 @@
 // members' rows form the batch of two products:
 //   C = X Phi^T   ([members x N] . [N x K]),   Y = (C W) Phi   ([members x K] . [K x N]).
 // One workgroup takes one slot and a block of 16 members; the 16 x N rows
-// sit in LDS, the basis is read from a cos/sin table in LDS as the
-// operands are formed, the forward product's accumulators stay in registers
-// until every wave is done with X, and the backward product writes Y (FP64)
-// to the member's second staging area.  k_filter then rescales and applies
+// sit in LDS, the basis operands are read from matrices built once per
+// handle (Phi[j][n] for the forward product, Phi^T[n][j] for the backward
+// one, so that a lane's operand elements are contiguous: one 16-byte load
+// for eight BF16 values), the forward product's accumulators stay in
+// registers until every wave is done with X, and the backward product writes
+// Y (FP64) to the member's second staging area.  k_filter then rescales and applies
 // the phase epilogue to those rows exactly as to its own FFT rows.
+//
+// The basis matrices (mm_basis_bytes, filled by mm_basis_fill on the host)
+// hold phi_j(n) for j < KB = N + 2 rounded up to 32 and n < N (zero up to N
+// rounded up to 32), in FP32, or as the high and low BF16 parts of the FP32
+// value; the columns past a row's K are masked out of C.
 //
 // MFMA lane layouts (checked on the device): v_mfma_f32_16x16x4_f32 takes
 // A[i][k] from lane i + 16k and B[k][j] from lane j + 16k;
@@ -31,8 +38,11 @@
 // j + 16q.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "gd_common.hpp"
 #include "gd_device.hpp"
@@ -57,10 +67,12 @@ __host__ __device__ inline int mm_ld(int N, int K) {
 
 // The tile holds FP32 values, or the high and low BF16 parts of each value
 // (two arrays of BF16): 4 bytes per element either way.
+// The tile holds FP32 values, or the high and low BF16 parts of each value
+// (two arrays of BF16): 4 bytes per element either way.
 size_t filter_mm_lds_bytes(int N, bool bf16) {
   (void)bf16;
   const int K = N + 2;  // the widest kept basis
-  return (size_t)N * 8 + (size_t)kMmRows * mm_ld(N, K) * 4;
+  return (size_t)kMmRows * mm_ld(N, K) * 4;
 }
 
 // x = hi + lo with hi = bf16(x), lo = bf16(x - hi): 16 mantissa bits.
@@ -68,17 +80,6 @@ __device__ __forceinline__ void mm_split(float x, __bf16 &hi, __bf16 &lo) {
   hi = (__bf16)x;
   lo = (__bf16)(x - (float)hi);
 }
-
-// Basis function j at longitude index m = (k n) mod N (k its wavenumber)
-// from the cos/sin table: j = 0 and odd j are cosines, even j > 0 sines;
-// j >= K is zero (padding).
-__device__ __forceinline__ float mm_basis(const float2 *tw, int j, int K, int m) {
-  const float2 t = tw[m];
-  const float v = (j > 0 && !(j & 1)) ? t.y : t.x;
-  return j < K ? v : 0.0f;
-}
-
-__device__ __forceinline__ int mm_wrap(int m, int N) { return m >= N ? m - N : m; }
 
 template <bool BF16>
 __global__ void __launch_bounds__(kBlock) k_filter_mm(const Args a) {
@@ -94,18 +95,17 @@ __global__ void __launch_bounds__(kBlock) k_filter_mm(const Args a) {
   const int N = a.N, P = a.pitch, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int c = (ss.kept - 2) / 2, K = ss.kept;  // kept basis: 2c + 2 functions
   const int Kp = mm_kpad(K), Np = mm_npad(N), LD = mm_ld(N, K);
+  const int KB = mm_kpad(N + 2);  // rows of the basis matrix (every kept basis fits)
   extern __shared__ __align__(16) unsigned char smem[];
-  float2 *tw = (float2 *)smem;
-  T *xs = (T *)(smem + (size_t)N * 8);
+  T *xs = (T *)smem;
   T *xl = xs + (size_t)kMmRows * mm_ld(N, N + 2);  // BF16 only
+  // basis matrices: Phi[j][n] (KB x Np) then Phi^T[n][j] (Np x KB); BF16:
+  // high parts then low parts of each
+  const T *phi = (const T *)a.mm_phi;
+  const T *phi_lo = phi + (size_t)KB * Np;
+  const T *phit = BF16 ? phi + (size_t)2 * KB * Np : phi + (size_t)KB * Np;
+  const T *phit_lo = phit + (size_t)KB * Np;
 
-  // cos/sin of 2 pi m/N for m = 0..N-1 (the table holds m <= N/2; the rest
-  // are conjugates)
-  for (int m = tid; m < N; m += kBlock) {
-    const int q = m <= N / 2 ? m : N - m;
-    const float cs = (float)a.twid[2 * q], sn = (float)a.twid[2 * q + 1];
-    tw[m] = make_float2(cs, m <= N / 2 ? sn : -sn);
-  }
   // the block's rows (zero past the last active member and past N)
   for (int e = tid; e < kMmRows * Np; e += kBlock) {
     const int b = e / Np, n = e - b * Np;
@@ -129,18 +129,8 @@ __global__ void __launch_bounds__(kBlock) k_filter_mm(const Args a) {
   v4f acc[kMmTpw];
 #pragma unroll
   for (int t = 0; t < kMmTpw; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-  // per tile: this lane's basis column j = (wv + 4t) 16 + col, of
-  // wavenumber k = (j + 1)/2 < N, and the table index m = (k n) mod N at the
-  // lane's first longitude, advanced by (k kStep) mod N per step
   auto col_j = [&](int t) { return (wv + 4 * t) * 16 + col; };
-  int mm[kMmTpw], inc[kMmTpw];
   constexpr int kStep = BF16 ? 32 : 4;
-#pragma unroll
-  for (int t = 0; t < kMmTpw; ++t) {
-    const int k = (col_j(t) + 1) >> 1;
-    mm[t] = (k * (BF16 ? 8 * q : q)) % N;
-    inc[t] = (k * kStep) % N;
-  }
   for (int n0 = 0; n0 < Np; n0 += kStep) {
     if (BF16) {
       v8bf ah, al;
@@ -152,38 +142,27 @@ __global__ void __launch_bounds__(kBlock) k_filter_mm(const Args a) {
 #pragma unroll
       for (int t = 0; t < kMmTpw; ++t) {
         if (wv + 4 * t < ntile) {
-          v8bf bh, bl;
-          const int j = col_j(t), dk = (j + 1) >> 1;
-          int m = mm[t];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            // longitudes past N carry zero rows, whatever the basis value
-            __bf16 h, l;
-            mm_split(mm_basis(tw, j, K, m), h, l);
-            bh[e] = h;
-            bl[e] = l;
-            m = mm_wrap(m + dk, N);
-          }
+          // longitudes past N carry zero rows
+          const size_t o = (size_t)col_j(t) * Np + n0 + 8 * q;
+          const v8bf bh = *(const v8bf *)(phi + o), bl = *(const v8bf *)(phi_lo + o);
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[t], 0, 0, 0);
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[t], 0, 0, 0);
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
         }
-        mm[t] = mm_wrap(mm[t] + inc[t], N);
       }
     } else {
       const float av = (float)xs[col * LD + n0 + q];
 #pragma unroll
       for (int t = 0; t < kMmTpw; ++t) {
         if (wv + 4 * t < ntile) {
-          const float bv = mm_basis(tw, col_j(t), K, mm[t]);
+          const float bv = (float)phi[(size_t)col_j(t) * Np + n0 + q];
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
         }
-        mm[t] = mm_wrap(mm[t] + inc[t], N);
       }
     }
   }
   __syncthreads();  // every wave is done with X
-  // C W into the tile: column j of row 4q + r
+  // C W into the tile: column j of row 4q + r; columns past K are zero
   const float wlast = (2 * (c + 1) == N) ? 1.0f / N : 2.0f / N;
 #pragma unroll
   for (int t = 0; t < kMmTpw; ++t) {
@@ -210,21 +189,8 @@ __global__ void __launch_bounds__(kBlock) k_filter_mm(const Args a) {
   const int nt_all = Np / 16;
   for (int nt0 = wv * 4; nt0 < nt_all; nt0 += 16) {
     v4f y[4];
-    int nn[4], nmod[4], mb[4], step[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      y[u] = v4f{0.f, 0.f, 0.f, 0.f};
-      const int n = (nt0 + u) * 16 + col;
-      nn[u] = n;
-      const int nm = n % N;
-      nmod[u] = nm;
-      // lane's first basis row j = 8q (bf16) or q (f32): wavenumber kb and
-      // the table index of kb n; per step the wavenumber grows by 16 (bf16)
-      // or 2 (f32)
-      const int kb = BF16 ? 4 * q : 0;
-      mb[u] = (kb * nm) % N;
-      step[u] = ((BF16 ? 16 : 2) * nm) % N;
-    }
+    for (int u = 0; u < 4; ++u) y[u] = v4f{0.f, 0.f, 0.f, 0.f};
     for (int j0 = 0; j0 < Kp; j0 += kStep) {
       if (BF16) {
         v8bf ah, al;
@@ -236,48 +202,30 @@ __global__ void __launch_bounds__(kBlock) k_filter_mm(const Args a) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           if (nt0 + u >= nt_all) continue;
-          // rows j = j0 + 8q + e: wavenumbers kb, kb+1, kb+1, kb+2, kb+2,
-          // kb+3, kb+3, kb+4 (cos, then sin and cos pairs)
-          const int nm = nmod[u];
-          int m1 = mm_wrap(mb[u] + nm, N), m2 = mm_wrap(m1 + nm, N), m3 = mm_wrap(m2 + nm, N);
-          const int m4 = mm_wrap(m3 + nm, N);
-          const int jb = j0 + 8 * q;
-          const int mi[8] = {mb[u], m1, m1, m2, m2, m3, m3, m4};
-          v8bf bh, bl;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            __bf16 h, l;
-            mm_split(mm_basis(tw, jb + e, K, mi[e]), h, l);
-            bh[e] = h;
-            bl[e] = l;
-          }
+          const size_t o = (size_t)((nt0 + u) * 16 + col) * KB + j0 + 8 * q;
+          const v8bf bh = *(const v8bf *)(phit + o), bl = *(const v8bf *)(phit_lo + o);
           y[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, y[u], 0, 0, 0);
           y[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, y[u], 0, 0, 0);
           y[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, y[u], 0, 0, 0);
-          mb[u] = mm_wrap(mb[u] + step[u], N);
         }
       } else {
         const float av = (float)xs[col * LD + j0 + q];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           if (nt0 + u >= nt_all) continue;
-          // row j = j0 + q: wavenumber j0/2 + (q + 1)/2 (q = 0: sin, or 1 at
-          // j = 0; q = 1: cos; q = 2: sin; q = 3: cos)
-          const int nm = nmod[u];
-          const int m = (q == 0) ? mb[u] : (q <= 2 ? mm_wrap(mb[u] + nm, N) : mm_wrap(mm_wrap(mb[u] + nm, N) + nm, N));
-          const float bv = mm_basis(tw, j0 + q, K, m);
+          const float bv = (float)phit[(size_t)((nt0 + u) * 16 + col) * KB + j0 + q];
           y[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, y[u], 0, 0, 0);
-          mb[u] = mm_wrap(mb[u] + step[u], N);
         }
       }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      if (nt0 + u >= nt_all || nn[u] >= N) continue;
+      const int n = (nt0 + u) * 16 + col;
+      if (nt0 + u >= nt_all || n >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + 4 * q + r;
-        if (m < a.nmem) member_at(a, m).sy()[(size_t)slot * P + nn[u]] = (double)y[u][r];
+        if (m < a.nmem) member_at(a, m).sy()[(size_t)slot * P + n] = (double)y[u][r];
       }
     }
   }
@@ -293,6 +241,61 @@ static hipError_t launch_mm_t(const Args &a, hipStream_t s) {
 hipError_t launch_filter_mm(const Args &a, bool bf16, hipStream_t s) {
   if (a.ntiles <= 0) return hipSuccess;
   return bf16 ? launch_mm_t<true>(a, s) : launch_mm_t<false>(a, s);
+}
+
+size_t mm_basis_bytes(int N) { return (size_t)2 * mm_kpad(N + 2) * mm_npad(N) * 4; }
+
+// BF16 with round to nearest even, as the device conversion does.
+static uint16_t mm_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static float mm_bf16f(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+void mm_basis_fill(int N, bool bf16, std::vector<unsigned char> &out) {
+  const int KB = mm_kpad(N + 2), Np = mm_npad(N);
+  // cos, sin of 2 pi m/N in extended precision, rounded once
+  std::vector<double> cs(N), sn(N);
+  for (int m = 0; m < N; ++m) {
+    const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)N;
+    cs[m] = (double)cosl(ang);
+    sn[m] = (double)sinl(ang);
+  }
+  std::vector<float> phi((size_t)KB * Np, 0.0f);
+  for (int j = 0; j < KB; ++j) {
+    const long long k = (j + 1) >> 1;
+    const bool sine = j > 0 && !(j & 1);
+    for (int n = 0; n < N; ++n) {
+      const int m = (int)((k * n) % N);
+      phi[(size_t)j * Np + n] = (float)(sine ? sn[m] : cs[m]);
+    }
+  }
+  out.assign(mm_basis_bytes(N), 0);
+  if (!bf16) {
+    float *a = (float *)out.data(), *t = a + (size_t)KB * Np;
+    for (int j = 0; j < KB; ++j)
+      for (int n = 0; n < Np; ++n) {
+        a[(size_t)j * Np + n] = phi[(size_t)j * Np + n];
+        t[(size_t)n * KB + j] = phi[(size_t)j * Np + n];
+      }
+    return;
+  }
+  uint16_t *h = (uint16_t *)out.data(), *l = h + (size_t)KB * Np, *th = l + (size_t)KB * Np,
+           *tl = th + (size_t)KB * Np;
+  for (int j = 0; j < KB; ++j)
+    for (int n = 0; n < Np; ++n) {
+      const float x = phi[(size_t)j * Np + n];
+      const uint16_t hi = mm_bf16(x), lo = mm_bf16(x - mm_bf16f(hi));
+      h[(size_t)j * Np + n] = th[(size_t)n * KB + j] = hi;
+      l[(size_t)j * Np + n] = tl[(size_t)n * KB + j] = lo;
+    }
 }
 
 hipError_t configure_filter_mm(int max_lds) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 #include "gd_device.hpp"
 
@@ -49,6 +50,10 @@ size_t phase_lds_bytes(int N, int L, int prec);
 hipError_t launch_filter_mm(const Args &a, bool bf16, hipStream_t s);
 hipError_t configure_filter_mm(int max_lds);
 size_t filter_mm_lds_bytes(int N, bool bf16);
+// the basis matrices of k_filter_mm (Phi and Phi^T, FP32 or BF16 high and
+// low parts), built once per handle
+size_t mm_basis_bytes(int N);
+void mm_basis_fill(int N, bool bf16, std::vector<unsigned char> &out);
 hipError_t configure_phase(int max_lds);
 
 }  // namespace gd

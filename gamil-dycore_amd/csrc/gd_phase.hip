@@ -119,11 +119,17 @@ __device__ __forceinline__ void near_tile(const Args &a, const FiltRow fr, unsig
   filter_row<CT, KP, true>(a, mb, fr, tw, b0, b1, red, PASS == 2 && fr.comp == CDGD && a.phase <= PH_T);
 }
 
-// Five waves per SIMD (at most 96 VGPRs): with the LDS of an FP64 tile that
-// is five tiles per CU, so a 720x361 run's phase fits in one round of
-// resident workgroups.
+// Four longitudes per thread (rows up to 1024): five waves per SIMD (at most
+// 96 VGPRs); with the LDS of an FP64 tile that is five tiles per CU, so a
+// 720x361 run's phase fits in one round of resident workgroups.  Eight
+// (rows up to 2048): the twiddle table and row buffer of a 1440-point row
+// hold the LDS to three tiles per CU, so three waves per SIMD.
+template <int KP>
+constexpr int phase_waves() { return KP == 4 ? 5 : 3; }
+
 template <typename CT, int PASS, bool IN_FULL, int KP>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) k_phase(const Args a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(phase_waves<KP>(), 8)))
+k_phase(const Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[4 * 6];
   const int b = blockIdx.x;
@@ -200,7 +206,8 @@ __device__ __forceinline__ void dnear_tile(const Args &a, const FiltRow fr, unsi
 }
 
 template <typename CT, int KP>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) k_dphase(const Args a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(phase_waves<KP>(), 8)))
+k_dphase(const Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[4 * 6];
   const int b = blockIdx.x;
@@ -216,14 +223,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 
 }
 
 hipError_t launch_dphase(const Args &a, int prec, hipStream_t s) {
-  if (a.N > 4 * kBlock) return hipErrorInvalidValue;
+  if (a.N > 8 * kBlock) return hipErrorInvalidValue;
   const int rows = ((a.ntiles + kXcd - 1) / kXcd) * kXcd;
   dim3 g(a.nnear_pad + rows), b(kBlock);
   const size_t lds = dphase_lds(a.N, a.fft_L);
-  if (prec == 0)
-    hipLaunchKernelGGL((k_dphase<double, 4>), g, b, lds, s, a);
-  else
-    hipLaunchKernelGGL((k_dphase<float, 4>), g, b, lds, s, a);
+  const bool k8 = a.N > 4 * kBlock;
+  if (prec == 0) {
+    if (k8) hipLaunchKernelGGL((k_dphase<double, 8>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_dphase<double, 4>), g, b, lds, s, a);
+  } else {
+    if (k8) hipLaunchKernelGGL((k_dphase<float, 8>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_dphase<float, 4>), g, b, lds, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -247,7 +258,9 @@ static hipError_t launch_phase_k(const Args &a, int pass, bool in_full, hipStrea
 }
 
 hipError_t launch_phase(const Args &a, int prec, int pass, bool in_full, hipStream_t s) {
-  if (a.N > 4 * kBlock) return hipErrorInvalidValue;
+  if (a.N > 8 * kBlock) return hipErrorInvalidValue;
+  if (a.N > 4 * kBlock)
+    return prec == 0 ? launch_phase_k<double, 8>(a, pass, in_full, s) : launch_phase_k<float, 8>(a, pass, in_full, s);
   return prec == 0 ? launch_phase_k<double, 4>(a, pass, in_full, s) : launch_phase_k<float, 4>(a, pass, in_full, s);
 }
 
@@ -267,10 +280,13 @@ hipError_t configure_phase(int max_lds) {
   hipError_t e;
   if ((e = configure_phase_k<double, 4>(max_lds)) != hipSuccess) return e;
   if ((e = configure_phase_k<float, 4>(max_lds)) != hipSuccess) return e;
-  if ((e = hipFuncSetAttribute((const void *)k_dphase<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               max_lds)) != hipSuccess)
-    return e;
-  return hipFuncSetAttribute((const void *)k_dphase<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+  if ((e = configure_phase_k<double, 8>(max_lds)) != hipSuccess) return e;
+  if ((e = configure_phase_k<float, 8>(max_lds)) != hipSuccess) return e;
+  const void *fns[] = {(const void *)k_dphase<double, 4>, (const void *)k_dphase<float, 4>,
+                       (const void *)k_dphase<double, 8>, (const void *)k_dphase<float, 8>};
+  for (const void *f : fns)
+    if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds)) != hipSuccess) return e;
+  return hipSuccess;
 }
 
 }  // namespace gd

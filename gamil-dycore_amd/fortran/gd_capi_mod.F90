@@ -217,6 +217,16 @@ module gd_capi_mod
       import :: c_ptr
     end function gd_io_last_error
 
+    integer(c_int) function gd_autotune(h, steps, cache_path, fused, upd_wpm, ms_per_step) &
+                                        bind(C, name='gd_autotune')
+      import :: c_int, c_int32_t, c_double, c_ptr, c_char
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: steps
+      character(kind=c_char), intent(in) :: cache_path(*)
+      integer(c_int32_t), intent(out) :: fused, upd_wpm
+      real(c_double), intent(out) :: ms_per_step
+    end function gd_autotune
+
   end interface
 
 contains

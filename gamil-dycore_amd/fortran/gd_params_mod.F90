@@ -9,6 +9,9 @@
 !   ensemble_members        number of device-resident runs of this config
 !   device                  HIP device ordinal
 !   log_every               print the per-step diagnostics line every n steps
+!   autotune_steps          > 0: time that many steps of each launch shape that cannot
+!                           change a result and keep the fastest (gd_autotune) before the run
+!   autotune_cache          file of earlier autotuner results ('' : none)
 !   write_output            .false.: no history or restart files (timing runs)
 !   output_dir              directory of the history and restart files
 !                           (default: the working directory, as the reference)
@@ -79,6 +82,8 @@ module gd_params_mod
   integer :: ensemble_members = 1
   integer :: device = 0
   integer :: log_every = 1
+  integer :: autotune_steps = 0
+  character(256) :: autotune_cache = ''
   logical :: write_output = .true.
   character(256) :: output_dir = ''
   integer :: days = 0
@@ -95,7 +100,7 @@ module gd_params_mod
     uv_adv_upwind_lat_beta, use_zonal_tend_filter, zonal_tend_filter_cutoff_wavenumber, &
     use_diffusion, diffusion_order, diffusion_coef, &
     zonal_tend_filter_lat0, precision, ensemble_members, device, log_every, days, &
-    write_output, output_dir, &
+    autotune_steps, autotune_cache, write_output, output_dir, &
     use_zonal_reduce, reduce_adv_lon, zonal_reduce_factors, use_reduce_tend_smooth
 
 contains

@@ -389,9 +389,10 @@ int64_t gd_device_bytes(const gd_handle *h);
 enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2, GD_TIME_PHASE = 3, GD_TIME_FILTER_MM = 4 };
 
 /* 1 when the handle's step, at its current active-member count, runs each
- * tendency phase as one fused launch, else 0.  Phases are fused for
- * num_lon <= 1024 with filtered rows while active members x num_lat x
- * ceil(num_lon/254) <= 24576 row tiles; GD_FUSED=0 or 1 in the environment
+ * tendency phase as one fused launch, else 0.  By default phases are fused
+ * with filtered rows while active members x num_lat x ceil(num_lon/254) row
+ * tiles stay within 24576 (num_lon <= 1024) or 6144 (num_lon <= 2048), and
+ * gd_autotune may choose either form; GD_FUSED=0 or 1 in the environment
  * when the handle is created forces separate or fused launches.  No
  * counterpart in the reference. */
 int gd_fused_phases(const gd_handle *h);

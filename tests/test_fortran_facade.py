@@ -121,3 +121,27 @@ def test_driver_rossby_haurwitz(exe, tmp_path):
     r = subprocess.run([exe, str(nml)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "final total_mass" in r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_autotune_gives_the_same_run(exe, tmp_path):
+    """autotune_steps in &dycore_params runs gd_autotune on the handed-over
+    state before the loop: the per-step log lines (mass, energy, beta printed
+    to 21 digits) are those of the untuned run; the matrix-core filter mode is
+    reachable from the namelist too."""
+    base = SMALL.format(case="balanced_sh")
+    runs = {}
+    for name, extra in (("plain", ""), ("tuned", "autotune_steps = 5\nautotune_cache = 'tune.txt'\n"),
+                        ("mfma", "precision = 'mixed_mfma'\n")):
+        text = base.replace("log_every = 1\n", "log_every = 1\n" + extra)
+        nml = tmp_path / f"{name}.nml"
+        nml.write_text(text)
+        print(f"driver run: {name}", flush=True)
+        r = subprocess.run([exe, str(nml)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert r.returncode == 0, r.stdout + r.stderr
+        runs[name] = r.stdout
+    steps = lambda out: [ln for ln in out.splitlines() if ln.startswith(" => ")]
+    assert steps(runs["tuned"]) == steps(runs["plain"])
+    assert "autotune: fused = " in runs["tuned"]
+    assert (tmp_path / "tune.txt").read_text().count("N=72 M=37") == 1
+    assert len(steps(runs["mfma"])) == 73

@@ -75,6 +75,9 @@ CASES = [
     # swizzle, three segments per row) and the 180-longitude plan
     (720, 361, 120.0, 3, dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=2.5e4)),
     (180, 91, 480.0, 3, dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=1e5)),
+    # eight longitudes per thread in the filter tiles (1440: the compile-time
+    # 720-point plan, six segments per row)
+    (1440, 61, 60.0, 2, dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_coef=6.25e3)),
     (72, 37, 900.0, 4, dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_order=4,
                             diffusion_coef=2e5 * 4e10)),
 ]
@@ -100,12 +103,23 @@ def test_fused_phase_ensemble_active_subset(gd):
                 run(gd, False, N, M, 900.0, 4, members=3, active=2, **kw))
 
 
-def test_fused_phase_off_above_1024_longitudes(gd):
-    """num_lon > 1024 keeps k_tend + k_filter even when fusing is asked for."""
-    N, M = 1026, 17
+def test_fused_phase_off_above_2048_longitudes(gd):
+    """num_lon > 2048 keeps k_tend + k_filter even when fusing is asked for;
+    past 1024 the phases fuse by default up to 6144 row tiles."""
+    N, M = 2050, 17
     kw = dict(split_scheme="csp2", subcycles=2, zonal_tend_filter_cutoff_wavenumber=gd.cutoff_from_lat0(N, M, 60.0))
     fam = families(gd, N, M, 30.0, True, **kw)
     assert "phase" not in fam and "filter" in fam
+    old = os.environ.pop("GD_FUSED", None)
+    try:
+        N, M = 1440, 37
+        with gd.Dycore(N, M, 60.0, members=30, zonal_tend_filter_cutoff_wavenumber=gd.cutoff_from_lat0(N, M, 60.0)) as d:
+            assert not d.fused_phases()  # 30 x 37 x 6 = 6,660 row tiles
+            d.set_active_members(27)
+            assert d.fused_phases()      # 5,994
+    finally:
+        if old is not None:
+            os.environ["GD_FUSED"] = old
 
 
 def test_fused_phase_rule_by_launch_size(gd):
