@@ -158,6 +158,10 @@ def _time_steps(d, steps, warmup):
     return time.perf_counter() - t0
 
 
+# the phase launch of each form that fused_phases() reports
+KERNEL_OF_FORM = {0: "k_tend", 1: "k_phase", 2: "k_wphase"}
+
+
 def _kernel_us(d, reps):
     """Replayed duration of the fast-pass second-predictor phase launch:
     k_phase when the handle fuses phases, else k_tend."""
@@ -170,7 +174,7 @@ def _tend_roofline(N, M, members, us, fused):
     members (its algorithmic bytes over its replayed duration)."""
     nbytes = tend_launch_bytes(N, M, "mixed") * members
     achieved = nbytes / (us * 1e-6) / 1e9
-    kname = "k_phase" if fused else "k_tend"
+    kname = KERNEL_OF_FORM[fused]
     return {"bound": "hbm", "kernel": f"{kname}<float, fast pass> (second predictor)",
             "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": None, "bytes_per_launch": int(nbytes),
@@ -559,7 +563,7 @@ def main():
     nbytes = tend_launch_bytes(N, M, "mixed")
     avg_s = mixed["kern_us"] * 1e-6
     achieved = nbytes / avg_s / 1e9
-    kname = "k_phase" if mixed["fused"] else "k_tend"
+    kname = KERNEL_OF_FORM[mixed["fused"]]
     roofline = {"bound": "hbm",
                 "kernel": f"{kname}<float, fast pass> (second predictor" +
                           (": tendency + filter rows + epilogue)" if mixed["fused"] else ")"),

@@ -544,18 +544,21 @@ class Dycore:
         write_restart(path, u, v, gd, self._ghs[member], **info)
 
     def fused_phases(self):
-        """True when each tendency phase runs as one fused launch."""
-        return bool(lib().gd_fused_phases(self.h))
+        """Form of the tendency phase launches: 0 a tendency launch then a
+        filter launch, 1 one fused launch (k_phase), 2 one fused launch with
+        a workgroup per row (k_wphase).  Nonzero means fused."""
+        return int(lib().gd_fused_phases(self.h))
 
     def autotune(self, steps=40, cache=None):
         """Pick the fastest launch shapes for the current active-member count
         (gd_autotune; results stay bit for bit those of the defaults).
-        Returns {'fused', 'upd_wpm', 'ms_per_step', 'cache'}."""
+        Returns {'fused', 'form', 'upd_wpm', 'ms_per_step', 'cache'}: 'form' as
+        fused_phases() reports it, 'fused' whether that form is fused."""
         f, w, ms = C.c_int32(0), C.c_int32(0), C.c_double(0.0)
         _check(lib().gd_autotune(self.h, int(steps), None if cache is None else str(cache).encode(),
                                  C.byref(f), C.byref(w), C.byref(ms)), self.h)
-        return {"fused": bool(f.value), "upd_wpm": int(w.value), "ms_per_step": float(ms.value),
-                "cache": cache}
+        return {"fused": bool(f.value), "form": int(f.value), "upd_wpm": int(w.value),
+                "ms_per_step": float(ms.value), "cache": cache}
 
     def device_bytes(self):
         return int(lib().gd_device_bytes(self.h))

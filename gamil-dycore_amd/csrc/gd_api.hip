@@ -69,9 +69,12 @@ struct gd_handle {
   std::vector<int> fft_rad;
   // fused phase launches (k_phase): the filter rows inside the tendency
   // launch; the row tiles cover the row ranges far_lo/far_cnt.  fused_ok:
-  // the shape allows them; fused_mode: -1 by launch size, 0 never, 1 always
-  bool fused_ok = false;
+  // the shape allows them; wide_ok: also one workgroup per row (k_wphase);
+  // fused_mode: -1 by launch size, 0 never, 1 always k_phase, 2 always
+  // k_wphase where wide_ok
+  bool fused_ok = false, wide_ok = false;
   int fused_mode = -1;
+  int ncu = 256;  // compute units of the device
   int nfar_rng = 0, nfar_rows = 0;
   int far_lo[kMaxFarRng] = {}, far_cnt[kMaxFarRng] = {};
   int32_t *dstep = nullptr;
@@ -86,8 +89,9 @@ struct gd_handle {
   // their filter workgroups (slot = position in the scratch array)
   double *scratch = nullptr;
   std::string scratch_err;
-  // autotuned launch shapes per active-member count (gd_autotune): fused (1)
-  // or separate (0) phase launches, k_update workgroups per member
+  // autotuned launch shapes per active-member count (gd_autotune): phase
+  // launch form (0 separate, 1 k_phase, 2 k_wphase), k_update workgroups
+  // per member
   std::map<int, int> tuned_fused, tuned_wpm;
   std::vector<int> prow[2];
   FiltRow *dprow[2] = {nullptr, nullptr};
@@ -234,26 +238,39 @@ constexpr int64_t kFusedMaxTiles = 24576;
 // larger launches are left to the autotuner.
 constexpr int64_t kFusedMaxTilesWide = 6144;
 
-bool use_fused(const gd_handle *h) {
-  if (!h->fused_ok || h->fused_mode == 0) return false;
-  if (h->fused_mode == -1) {
-    auto it = h->tuned_fused.find(h->nact);
-    if (it != h->tuned_fused.end()) return it->second != 0;
+// Form of the phase launches: 0 k_tend + k_filter, 1 k_phase, 2 k_wphase.
+// k_wphase (rows of two segments) by default while its workgroups -- one per
+// filter row and one per far row -- fit two per CU: measured on MI355X
+// (mixed, one or two runs of 288x145, 360x181, 480x241) 5-10% faster than
+// k_phase, and 4-20% slower once they take a second round.
+int phase_form(const gd_handle *h) {
+  if (!h->fused_ok || h->fused_mode == 0) return 0;
+  if (h->fused_mode == 1) return 1;
+  if (h->fused_mode == 2) return h->wide_ok ? 2 : 1;
+  auto it = h->tuned_fused.find(h->nact);
+  if (it != h->tuned_fused.end()) return it->second == 2 && !h->wide_ok ? 1 : it->second;
+  if (h->wide_ok) {
+    const int64_t wgs = (int64_t)h->nact * ((int64_t)(h->frows.size() / h->nmem) + h->nfar_rows);
+    if (wgs <= 2 * (int64_t)h->ncu) return 2;
   }
   const int64_t tiles = (int64_t)h->nact * h->M * ((h->N + kSeg - 1) / kSeg);
-  return h->fused_mode == 1 || tiles <= (h->N <= 4 * kBlock ? kFusedMaxTiles : kFusedMaxTilesWide);
+  return tiles <= (h->N <= 4 * kBlock ? kFusedMaxTiles : kFusedMaxTilesWide) ? 1 : 0;
 }
+
+bool use_fused(const gd_handle *h) { return phase_form(h) != 0; }
 
 // One tendency phase (a.phase) of the active members with its filter rows:
 // one k_phase launch, or k_tend then k_filter.
 hipError_t enqueue_phase(gd_handle *h, Args a, int pass, bool in_full) {
   a.ntiles = h->nact * h->M * a.nseg;
-  if (!use_fused(h)) {
+  const int form = phase_form(h);
+  if (form == 0) {
     hipError_t e = run(h, "tend", [&] { return launch_tend(a, h->prec, pass, in_full, h->stream); });
     if (e != hipSuccess) return e;
     return enqueue_filter(h, a);
   }
   set_fused(h, a, h->nact);
+  if (form == 2) return run(h, "wphase", [&] { return launch_wphase(a, h->prec, pass, in_full, h->stream); });
   return run(h, "phase", [&] { return launch_phase(a, h->prec, pass, in_full, h->stream); });
 }
 
@@ -706,10 +723,14 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
     h->nfar_rng = nr;
     h->nfar_rows = (int)far.size();
     const char *env = std::getenv("GD_FUSED");
-    h->fused_mode = env && env[0] == '0' ? 0 : (env && env[0] == '1' ? 1 : -1);
+    h->fused_mode = env && env[0] >= '0' && env[0] <= '2' ? env[0] - '0' : -1;
     // (the MFMA filter runs as its own launch between k_tend and k_filter)
     h->fused_ok = fits && !h->frows.empty() && h->N <= 8 * kBlock && !h->filt_mm &&
                   phase_lds_bytes(h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
+    const int nseg = (h->N + kSeg - 1) / kSeg;
+    h->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    h->wide_ok = h->fused_ok && nseg == 2 &&
+                 wphase_lds_bytes(nseg, h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
   }
   if (h->filt_mm && (h->N < 64 || h->N > 1440 || filter_mm_lds_bytes(h->N, h->filt_mm == 2) > prop.sharedMemPerBlock)) {
     delete h;
@@ -1112,7 +1133,7 @@ int gd_copy_member(gd_handle *h, int32_t dst, int32_t src) {
   return GD_OK;
 }
 
-int gd_fused_phases(const gd_handle *h) { return h && use_fused(h) ? 1 : 0; }
+int gd_fused_phases(const gd_handle *h) { return h ? phase_form(h) : 0; }
 
 int gd_autotune(gd_handle *h, int32_t steps, const char *cache_path, int32_t *fused_out, int32_t *wpm_out,
                 double *ms_out) {
@@ -1127,15 +1148,16 @@ int gd_autotune(gd_handle *h, int32_t steps, const char *cache_path, int32_t *fu
   double bms = 0.0;
   if (cache_path && !cache_path[0]) cache_path = nullptr;  // "" : no cache
   bool cached = cache_path && tune_lookup(cache_path, key, bf, bw, bms);
-  if (cached && (bw < 1 || bw > h->M || (bf && !h->fused_ok))) cached = false;
+  if (cached && (bw < 1 || bw > h->M || bf < 0 || bf > 2 || (bf && !h->fused_ok) || (bf == 2 && !h->wide_ok)))
+    cached = false;
   if (!cached) {
     // fused or separate phase launches (both give the same bits; a forced
     // GD_FUSED keeps its form) ...
     std::vector<int> fopts;
-    if (h->fused_mode == 0 || !h->fused_ok)
-      fopts = {0};
-    else if (h->fused_mode == 1)
-      fopts = {1};
+    if (h->fused_mode >= 0 || !h->fused_ok)
+      fopts = {phase_form(h)};
+    else if (h->wide_ok)
+      fopts = {0, 1, 2};
     else
       fopts = {0, 1};
     // ... and k_update rows per workgroup (each workgroup forms its member's
@@ -1241,6 +1263,7 @@ int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us) {
     if (kernel == GD_TIME_PHASE) {
       Args f = a;
       set_fused(h, f, h->nmem);
+      if (phase_form(h) == 2) return launch_wphase(f, h->prec, pass, false, h->stream);
       return launch_phase(f, h->prec, pass, false, h->stream);
     }
     if (kernel == GD_TIME_FILTER_MM) {

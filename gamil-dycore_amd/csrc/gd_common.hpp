@@ -22,11 +22,18 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
   return t < ntiles ? t : -1;
 }
 
-// Deterministic workgroup sum (fixed butterfly + fixed cross-wave order);
-// every thread returns the same value.
+// Thread index within a tile of kBlock threads.  A workgroup of the wide
+// phase launch holds several such tiles side by side (tile threadIdx.x /
+// kBlock); in every other launch the workgroup is one tile.
+__device__ __forceinline__ int ltid() { return (int)(threadIdx.x & (kBlock - 1)); }
+
+// Deterministic sum over the kBlock threads of a tile (fixed butterfly +
+// fixed cross-wave order); every thread of the tile returns the same value.
+// The barriers are the workgroup's, so every tile of a workgroup calls it
+// at the same point, each with its own `red`.
 __device__ __forceinline__ double block_sum(double v, double *red) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  const int w = threadIdx.x >> 6;
+  const int w = (threadIdx.x >> 6) & (kBlock / 64 - 1);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[w] = v;
   __syncthreads();
@@ -35,14 +42,14 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
   return s;
 }
 
-// Deterministic workgroup sums of K values at once (one pair of barriers);
-// `red` holds 4*K doubles.  Every thread returns the same sums.
+// Deterministic tile sums of K values at once (one pair of barriers);
+// `red` holds 4*K doubles.  Every thread of the tile returns the same sums.
 template <int K>
 __device__ __forceinline__ void block_sum_n(double (&v)[K], double *red) {
 #pragma unroll
   for (int q = 0; q < K; ++q)
     for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
-  const int w = threadIdx.x >> 6;
+  const int w = (threadIdx.x >> 6) & (kBlock / 64 - 1);
   __syncthreads();
   if ((threadIdx.x & 63) == 0)
 #pragma unroll

@@ -149,13 +149,15 @@ __device__ __forceinline__ unsigned swz(unsigned e) {
 // first pass of a radix-8 plan writes each butterfly's eight outputs to
 // consecutive points, a 128-byte stride across the lanes (eight-way bank
 // conflicts); swizzled, the lanes cover all banks.
-template <int R, bool SWI = false, bool SWO = false>
+// NT: threads of the workgroup taking part (butterfly j goes to thread j mod
+// NT); the butterflies' arithmetic does not depend on it.
+template <int R, bool SWI = false, bool SWO = false, int NT = kBlock>
 __device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L, int Ns, Tw tw,
                                          int tstride, bool inv) {
   const unsigned Lr = L / R;
   const int tws = (L / (Ns * R)) * tstride;
   const int rstep = (int)Lr * tstride;
-  for (unsigned j = threadIdx.x; j < Lr; j += kBlock) {
+  for (unsigned j = threadIdx.x; j < Lr; j += NT) {
     const unsigned k = j % (unsigned)Ns;
     double2 v[R];
 #pragma unroll
@@ -172,12 +174,13 @@ __device__ __forceinline__ void fft_pass(const double2 *src, double2 *dst, int L
 }
 
 // The same pass for any radix R (prime factors above 7), O(R^2) per butterfly.
+template <int NT = kBlock>
 __device__ __forceinline__ void fft_pass_any(const double2 *src, double2 *dst, int L, int Ns, int R, Tw tw,
                                              int tstride, bool inv) {
   const int Lr = L / R;
   const int tws = (L / (Ns * R)) * tstride;
   const int rstep = Lr * tstride;
-  for (int j = threadIdx.x; j < Lr; j += kBlock) {
+  for (int j = threadIdx.x; j < Lr; j += NT) {
     const int k = j % Ns;
     const int o = (j - k) * R + k;
     for (int q = 0; q < R; ++q) {
@@ -194,6 +197,7 @@ __device__ __forceinline__ void fft_pass_any(const double2 *src, double2 *dst, i
 
 // Runs the plan's passes starting from buffer `cur`; returns the buffer that
 // holds the result.  Ends with a barrier.
+template <int NT = kBlock>
 __device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const Args &a, Tw tw, bool inv) {
   const int L = a.fft_L, tstride = a.fft_half ? 2 : 1;
   int Ns = 1;
@@ -202,12 +206,12 @@ __device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const 
     const double2 *src = cur ? b1 : b0;
     double2 *dst = cur ? b0 : b1;
     switch (R) {
-      case 2: fft_pass<2>(src, dst, L, Ns, tw, tstride, inv); break;
-      case 3: fft_pass<3>(src, dst, L, Ns, tw, tstride, inv); break;
-      case 4: fft_pass<4>(src, dst, L, Ns, tw, tstride, inv); break;
-      case 5: fft_pass<5>(src, dst, L, Ns, tw, tstride, inv); break;
-      case 8: fft_pass<8>(src, dst, L, Ns, tw, tstride, inv); break;
-      default: fft_pass_any(src, dst, L, Ns, R, tw, tstride, inv); break;
+      case 2: fft_pass<2, false, false, NT>(src, dst, L, Ns, tw, tstride, inv); break;
+      case 3: fft_pass<3, false, false, NT>(src, dst, L, Ns, tw, tstride, inv); break;
+      case 4: fft_pass<4, false, false, NT>(src, dst, L, Ns, tw, tstride, inv); break;
+      case 5: fft_pass<5, false, false, NT>(src, dst, L, Ns, tw, tstride, inv); break;
+      case 8: fft_pass<8, false, false, NT>(src, dst, L, Ns, tw, tstride, inv); break;
+      default: fft_pass_any<NT>(src, dst, L, Ns, R, tw, tstride, inv); break;
     }
     __syncthreads();
     Ns *= R;
@@ -223,35 +227,35 @@ __device__ __forceinline__ int fft_run(double2 *b0, double2 *b1, int cur, const 
 // The first radix-8 pass of a length divisible by 8 writes swizzled and the
 // second pass reads swizzled (only between those two passes does a buffer
 // hold the swizzled layout).
-template <int L, int TS, int NS, bool SWI, int R, int... REST>
+template <int NT, int L, int TS, int NS, bool SWI, int R, int... REST>
 __device__ __forceinline__ int fft_fixed_p(double2 *b0, double2 *b1, int cur, Tw tw, bool inv) {
   constexpr bool SWO = NS == 1 && R == 8 && L % 8 == 0 && sizeof...(REST) > 0;
-  fft_pass<R, SWI, SWO>(cur ? b1 : b0, cur ? b0 : b1, L, NS, tw, TS, inv);
+  fft_pass<R, SWI, SWO, NT>(cur ? b1 : b0, cur ? b0 : b1, L, NS, tw, TS, inv);
   __syncthreads();
   if constexpr (sizeof...(REST) > 0)
-    return fft_fixed_p<L, TS, NS * R, SWO, REST...>(b0, b1, cur ^ 1, tw, inv);
+    return fft_fixed_p<NT, L, TS, NS * R, SWO, REST...>(b0, b1, cur ^ 1, tw, inv);
   else
     return cur ^ 1;
 }
 
-template <int L, int TS, int NS, int... RS>
+template <int NT, int L, int TS, int NS, int... RS>
 __device__ __forceinline__ int fft_fixed(double2 *b0, double2 *b1, int cur, Tw tw, bool inv) {
-  return fft_fixed_p<L, TS, NS, false, RS...>(b0, b1, cur, tw, inv);
+  return fft_fixed_p<NT, L, TS, NS, false, RS...>(b0, b1, cur, tw, inv);
 }
 
 // The plans of the even row lengths 720, 360 and 180 (fft_plan of their
 // half lengths) run fixed, and 1440 (half length 720 = 8 2 3 3 5) where the
 // row takes eight or more points per thread (KP); others run through
 // fft_run.
-template <int KP>
+template <int KP, int NT = kBlock>
 __device__ __forceinline__ int fft_any(double2 *b0, double2 *b1, int cur, const Args &a, Tw tw, bool inv) {
   if constexpr (KP >= 8) {
-    if (a.fft_half && a.fft_L == 720) return fft_fixed<720, 2, 1, 8, 2, 3, 3, 5>(b0, b1, cur, tw, inv);
+    if (a.fft_half && a.fft_L == 720) return fft_fixed<NT, 720, 2, 1, 8, 2, 3, 3, 5>(b0, b1, cur, tw, inv);
   }
-  if (a.fft_half && a.fft_L == 360) return fft_fixed<360, 2, 1, 8, 3, 3, 5>(b0, b1, cur, tw, inv);
-  if (a.fft_half && a.fft_L == 180) return fft_fixed<180, 2, 1, 4, 3, 3, 5>(b0, b1, cur, tw, inv);
-  if (a.fft_half && a.fft_L == 90) return fft_fixed<90, 2, 1, 2, 3, 3, 5>(b0, b1, cur, tw, inv);
-  return fft_run(b0, b1, cur, a, tw, inv);
+  if (a.fft_half && a.fft_L == 360) return fft_fixed<NT, 360, 2, 1, 8, 3, 3, 5>(b0, b1, cur, tw, inv);
+  if (a.fft_half && a.fft_L == 180) return fft_fixed<NT, 180, 2, 1, 4, 3, 3, 5>(b0, b1, cur, tw, inv);
+  if (a.fft_half && a.fft_L == 90) return fft_fixed<NT, 90, 2, 1, 2, 3, 3, 5>(b0, b1, cur, tw, inv);
+  return fft_run<NT>(b0, b1, cur, a, tw, inv);
 }
 
 // FFTPACK5 mask of cutoff c on the complex spectrum: wavenumbers 0..c kept,
@@ -268,9 +272,10 @@ __device__ __forceinline__ double2 fmask(double2 X, int k, int c) {
 // E'_k + i O'_k with E'_k = (Y_k + conj Y_{L-k})/2, O'_k = (Y_k - conj
 // Y_{L-k}) W_N^{-k}/2, scaled by 1/L for the unnormalised inverse.  Pairs
 // (k, L-k) are handled together; k = 0 carries X_0 and the Nyquist X_L.
+template <int NT = kBlock>
 __device__ __forceinline__ void real_mid(const double2 *s, double2 *d, int L, int c, Tw tw, double sc) {
   const int H = L / 2 + 1;
-  for (int k = threadIdx.x; k < H; k += kBlock) {
+  for (int k = threadIdx.x; k < H; k += NT) {
     if (k == 0) {
       const double2 z = s[0];
       const double X0 = z.x + z.y, XL = z.x - z.y;
@@ -296,8 +301,9 @@ __device__ __forceinline__ void real_mid(const double2 *s, double2 *d, int L, in
 }
 
 // Odd N: mask the full complex spectrum symmetrically, scaled by 1/N.
+template <int NT = kBlock>
 __device__ __forceinline__ void cplx_mid(const double2 *s, double2 *d, int L, int c, double sc) {
-  for (int k = threadIdx.x; k < L; k += kBlock) {
+  for (int k = threadIdx.x; k < L; k += NT) {
     const int kk = k <= L - k ? k : L - k;
     const double2 Y = fmask(s[k], kk, c);
     d[k] = make_double2(Y.x * sc, Y.y * sc);
@@ -464,6 +470,126 @@ __device__ __forceinline__ void filter_row(const Args &a, const Member &mb, cons
     } else if (tid < a.nseg) {
       pp[2 * tid] = 0.0;
       pp[2 * tid + 1] = 0.0;
+    }
+  }
+  __syncthreads();  // LDS free for the caller
+}
+
+// Sums of K values over the first kBlock threads of a workgroup of NT
+// threads, in block_sum_n's order (so with its bits); the other threads
+// pass through the barriers, and every thread returns the sums.
+template <int K>
+__device__ __forceinline__ void head_sum_n(double (&v)[K], double *red) {
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0 && w < kBlock / 64)
+#pragma unroll
+    for (int q = 0; q < K; ++q) red[w * K + q] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double s = red[q];
+    for (int k = 1; k < kBlock / 64; ++k) s += red[k * K + q];
+    v[q] = s;
+  }
+}
+
+// filter_row<CT, KP, true> (the row already in b0, a rescaled phase) for a
+// workgroup of NT threads with one longitude each (N <= NT): the loads, the
+// transforms' butterflies and the epilogue spread over all NT threads.  The
+// sums keep filter_row's order: thread t < kBlock adds the products of
+// longitudes t, t + kBlock, ... in turn (staged through scr, 2 NT doubles),
+// then head_sum_n reduces those kBlock partial sums as block_sum_n does, so
+// both give the same bits.  Ends with a barrier.
+template <typename CT, int NT>
+__device__ __forceinline__ void filter_row_w(const Args &a, const Member &mb, const FiltRow &fr, double2 *tw,
+                                             double2 *b0, double2 *b1, double *scr, double *red, bool zero) {
+  const int N = a.N, P = a.pitch, L = a.fft_L, l = threadIdx.x;
+  const double sc = 1.0 / (double)L;
+  const int comp = fr.comp;
+  const size_t o = (size_t)fr.row * P;
+  const int ystep = a.fft_half ? 1 : 2;
+  const bool on = l < N;
+  const double *wp = mb.st(a.s_in, comp == CDU ? FU : (comp == CDV ? FV : FGD)) + o;
+  const bool gdrow = comp == CDGD;
+  double w = 0.0, wg = 0.0, pre = 0.0;
+  if (on) {
+    if (!zero) {
+      w = wp[l];
+      if (gdrow) wg = mb.ghs()[o + l];
+    }
+    pre = phase_pre<CT>(a, mb, comp, o + l);
+  }
+  // the row values are read before the first pass; b0 is first written by
+  // the second pass, after a barrier
+  const double x = on && !zero ? ((const double *)b0)[(size_t)l * ystep] : 0.0;
+  const int nk = (N + kBlock - 1) / kBlock;  // longitudes per head thread
+  double yv = 0.0;
+  double s12[2] = {0.0, 0.0};
+  if (!zero) {
+    const Tw twt{tw, N};
+    int cur = fft_any<4, NT>(b0, b1, 0, a, twt, false);
+    if (gdrow) w = w + wg;
+    if (a.fft_half)
+      real_mid<NT>(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, twt, sc);
+    else
+      cplx_mid<NT>(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, sc);
+    __syncthreads();
+    cur = fft_any<4, NT>(b0, b1, cur ^ 1, a, twt, true);
+    const double *y = (const double *)(cur ? b1 : b0);
+    yv = on ? y[(size_t)l * ystep] : 0.0;
+    if (on) {
+      scr[2 * l] = x * w;
+      scr[2 * l + 1] = yv * w;
+    }
+    __syncthreads();
+    if (l < kBlock)
+      for (int k = 0; k < nk; ++k) {
+        const int e = l + k * kBlock;
+        if (e < N) {
+          s12[0] += scr[2 * e];
+          s12[1] += scr[2 * e + 1];
+        }
+      }
+    head_sum_n<2>(s12, red);
+  }
+  const double s1 = s12[0], s2 = s12[1];
+  // rows with |s1| <= filter_inner_product_threshold stay unfiltered
+  const bool filt = fabs(s1) > 1.0e-16;
+  double p1 = 0.0, p2 = 0.0;
+  if (on) {
+    const double T = filt ? yv * s1 / s2 : x;
+    phase_store<CT>(a, mb, comp, o + l, T, pre, fr.wrow, p1, p2);
+  }
+  if (a.phase == PH_C) {
+    // (scr was last read before head_sum_n's barriers)
+    if (on) {
+      scr[2 * l] = p1;
+      scr[2 * l + 1] = p2;
+    }
+    __syncthreads();
+    double p[2] = {0.0, 0.0};
+    if (l < kBlock)
+      for (int k = 0; k < nk; ++k) {
+        const int e = l + k * kBlock;
+        if (e < N) {
+          p[0] += scr[2 * e];
+          p[1] += scr[2 * e + 1];
+        }
+      }
+    head_sum_n<2>(p, red);
+    // the whole row's products go to segment 0 of its partials, the other
+    // segments of the row hold zero
+    double *pp = mb.part() + ((size_t)(fr.row * 3 + comp) * a.nseg) * 2;
+    if (l == 0) {
+      pp[0] = p[0];
+      pp[1] = p[1];
+    } else if (l < a.nseg) {
+      pp[2 * l] = 0.0;
+      pp[2 * l + 1] = 0.0;
     }
   }
   __syncthreads();  // LDS free for the caller

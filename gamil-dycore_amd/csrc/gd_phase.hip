@@ -46,9 +46,10 @@ __host__ __device__ inline size_t phase_lds_t(int N, int L) {
 // hold it in flight).  Each wave copies whole 64-entry blocks; a lane moves
 // one 16-byte entry, and the wave's block lands contiguously from the LDS
 // address it names.  The copy is complete after s_waitcnt vmcnt(0).
+template <int NT = kBlock>
 __device__ __forceinline__ void tw_to_lds(const Args &a, double2 *tw) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, np = tw_padded(a.N);
-  for (int base = w * 64; base < np; base += kBlock) {
+  for (int base = w * 64; base < np; base += NT) {
     __builtin_amdgcn_global_load_lds((const void *)(a.twid + 2 * (base + lane)),
                                      (__attribute__((address_space(3))) void *)(tw + base), 16, 0, 0);
   }
@@ -149,6 +150,107 @@ k_phase(const Args a) {
   const int t = (bb % kXcd) * per + bb / kXcd;
   if (t >= a.ntiles) return;
   tend_tile<CT, PASS, IN_FULL, true>(a, t, smem, red);
+}
+
+// ---------------------------------------------------------------------------
+// k_wphase: k_phase with one workgroup per row.  A workgroup holds S tiles
+// of kBlock threads side by side, S = the row's segment count.  Its filter
+// tiles evaluate the S segments of their row at once (one per tile, each
+// with its own stencil rows) instead of one after another, and filter the
+// row with one longitude per thread (filter_row_w); its row tiles are the S
+// segment tiles of one far row, which take the same barriers.  Tile t's
+// arithmetic is that of k_phase, so both launches give the same bits.
+//
+// Launched for rows of two segments (255 to 508 longitudes), where a
+// workgroup is eight waves and two fit a CU at 96 VGPRs: a 360x181 run steps
+// 10% faster than with k_phase.  Rows of three segments measured slower (a
+// 720x361 run at 0.80 to 0.85 of k_phase): one twelve-wave workgroup per CU
+// at 96 VGPRs leaves the row workgroups a second round, and two per CU
+// (80 VGPRs) spill.
+//
+// LDS: S stencil regions of wphase_region bytes (region 0 later holds FFT
+// buffer b1, region 1 the filter's partial products), then the row buffer
+// b0 and the twiddle table.
+// ---------------------------------------------------------------------------
+
+template <typename CT>
+__host__ __device__ inline size_t wphase_region(int L) {
+  const size_t st = tend_stencil_bytes<CT>(), fb = (size_t)L * 16;
+  return align16(st > fb ? st : fb);
+}
+
+template <typename CT>
+__host__ __device__ inline size_t wphase_lds(int S, int N, int L) {
+  return (size_t)S * wphase_region<CT>(L) + (size_t)L * 16 + (size_t)tw_padded(N) * 16;
+}
+
+// Segment s of component CM of row j into the row buffer: tile s of the
+// workgroup, with the arithmetic of near_row's segment s.
+template <typename CT, int PASS, bool IN_FULL, int CM>
+__device__ __forceinline__ void wnear_seg(const Args &a, const Member &mb, const RowCoef &cj, int j, int s,
+                                          double *rowbuf, unsigned char *st, double *red) {
+  const int N = a.N, tid = ltid();
+  const RowNbr nb = row_nbr(a, j);
+  const int i0 = s * kSeg, W = min(N - i0, kSeg);
+  SegLoads ld;
+  seg_load<PASS, IN_FULL>(a, mb, j, i0, W, ld);
+  const TendPoint pt = seg_eval<CT, PASS, IN_FULL, CM>(a, cj, nb, j, i0, W, ld, false, 0.0,
+                                                       TendPoint{0, 0, 0, 0, 0, 0}, st, red);
+  if (tid < W) row_put(a, rowbuf, i0 + tid, CM == TC_DU ? pt.du : (CM == TC_DV ? pt.dv : pt.dgd));
+}
+
+template <typename CT, int PASS, bool IN_FULL, int S>
+__device__ __forceinline__ void wnear_tile(const Args &a, const FiltRow fr, unsigned char *smem, double *red) {
+  constexpr int NT = S * kBlock;
+  const int N = a.N, L = a.fft_L, s = threadIdx.x / kBlock;
+  const Member mb = member_at(a, fr.member);
+  const int j = fr.row;
+  const RowCoef cj = a.coef[j];
+  const size_t R = wphase_region<CT>(L);
+  double2 *b1 = (double2 *)smem;
+  double2 *b0 = (double2 *)(smem + S * R);
+  double2 *tw = b0 + L;
+  tw_to_lds<NT>(a, tw);
+  double *rowbuf = (double *)b0;
+  unsigned char *st = smem + s * R;
+  double *rd = red + s * 4 * 6;
+  if (fr.comp == CDU) {
+    wnear_seg<CT, PASS, IN_FULL, TC_DU>(a, mb, cj, j, s, rowbuf, st, rd);
+  } else if (fr.comp == CDV) {
+    wnear_seg<CT, PASS, IN_FULL, TC_DV>(a, mb, cj, j, s, rowbuf, st, rd);
+  } else if (PASS != 2 && (j == 0 || j == a.M - 1)) {
+    // a pole row's dgd is minus the pole flux at every longitude; each tile
+    // forms the same ring sum (dycore_mod.F90:572-596)
+    const double pole = pole_flux(a, mb, j, rd);
+    for (int l = threadIdx.x; l < N; l += NT) row_put(a, rowbuf, l, -pole);
+  } else if (PASS != 2) {
+    wnear_seg<CT, PASS, IN_FULL, TC_DGD>(a, mb, cj, j, s, rowbuf, st, rd);
+  }
+  // (the slow pass has no mass continuity term: its dgd rows are zero rows,
+  // which filter_row_w leaves unfiltered without reading b0)
+  __builtin_amdgcn_s_waitcnt(0);  // the twiddle copy has landed
+  __syncthreads();  // row and twiddles published; the stencil rows are free
+  filter_row_w<CT, NT>(a, mb, fr, tw, b0, b1, (double *)(smem + R), red, PASS == 2 && fr.comp == CDGD);
+}
+
+// Two workgroups of two tiles per CU (16 of the 20 waves that 96 VGPRs allow).
+template <typename CT, int PASS, bool IN_FULL, int S>
+__global__ void __launch_bounds__(S * kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_wphase(const Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[S * 4 * 6];
+  const int b = blockIdx.x;
+  if (b < a.nnear_pad) {
+    const int t = (b % kXcd) * (a.nnear_pad / kXcd) + b / kXcd;
+    if (t < a.nnear) wnear_tile<CT, PASS, IN_FULL, S>(a, a.frows[t], smem, red);
+    return;
+  }
+  // row workgroups: (member, far row), each XCD owning a contiguous band
+  const int bb = b - a.nnear_pad, per = (gridDim.x - a.nnear_pad) / kXcd;
+  const int r = (bb % kXcd) * per + bb / kXcd;
+  if (r * S >= a.ntiles) return;
+  const int s = threadIdx.x / kBlock;
+  tend_tile<CT, PASS, IN_FULL, true>(a, r * S + s, smem + s * wphase_region<CT>(a.fft_L), red + s * 4 * 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -264,6 +366,42 @@ hipError_t launch_phase(const Args &a, int prec, int pass, bool in_full, hipStre
   return prec == 0 ? launch_phase_k<double, 4>(a, pass, in_full, s) : launch_phase_k<float, 4>(a, pass, in_full, s);
 }
 
+template <typename CT, int S>
+static hipError_t launch_wphase_k(const Args &a, int pass, bool in_full, hipStream_t s) {
+  const int nrow = a.ntiles / S;  // (member, far row) workgroups
+  const int rows = ((nrow + kXcd - 1) / kXcd) * kXcd;
+  dim3 g(a.nnear_pad + rows), b(S * kBlock);
+  const size_t lds = wphase_lds<CT>(S, a.N, a.fft_L);
+#define GD_PH(P, F) hipLaunchKernelGGL((k_wphase<CT, P, F, S>), g, b, lds, s, a)
+  if (pass == 0) { if (in_full) GD_PH(0, true); else GD_PH(0, false); }
+  else if (pass == 1) { if (in_full) GD_PH(1, true); else GD_PH(1, false); }
+  else { if (in_full) GD_PH(2, true); else GD_PH(2, false); }
+#undef GD_PH
+  return hipGetLastError();
+}
+
+hipError_t launch_wphase(const Args &a, int prec, int pass, bool in_full, hipStream_t s) {
+  // one workgroup per row of two segments, every segment a tile
+  if (a.nseg != 2 || a.N > a.nseg * kBlock || a.ntiles % a.nseg != 0) return hipErrorInvalidValue;
+  return prec == 0 ? launch_wphase_k<double, 2>(a, pass, in_full, s) : launch_wphase_k<float, 2>(a, pass, in_full, s);
+}
+
+size_t wphase_lds_bytes(int nseg, int N, int L, int prec) {
+  return prec == 0 ? wphase_lds<double>(nseg, N, L) : wphase_lds<float>(nseg, N, L);
+}
+
+template <typename CT, int S>
+static hipError_t configure_wphase_k(int max_lds) {
+  const void *fns[] = {(const void *)k_wphase<CT, 0, true, S>,  (const void *)k_wphase<CT, 0, false, S>,
+                       (const void *)k_wphase<CT, 1, true, S>,  (const void *)k_wphase<CT, 1, false, S>,
+                       (const void *)k_wphase<CT, 2, true, S>,  (const void *)k_wphase<CT, 2, false, S>};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template <typename CT, int KP>
 static hipError_t configure_phase_k(int max_lds) {
   const void *fns[] = {(const void *)k_phase<CT, 0, true, KP>,  (const void *)k_phase<CT, 0, false, KP>,
@@ -282,6 +420,8 @@ hipError_t configure_phase(int max_lds) {
   if ((e = configure_phase_k<float, 4>(max_lds)) != hipSuccess) return e;
   if ((e = configure_phase_k<double, 8>(max_lds)) != hipSuccess) return e;
   if ((e = configure_phase_k<float, 8>(max_lds)) != hipSuccess) return e;
+  if ((e = configure_wphase_k<double, 2>(max_lds)) != hipSuccess) return e;
+  if ((e = configure_wphase_k<float, 2>(max_lds)) != hipSuccess) return e;
   const void *fns[] = {(const void *)k_dphase<double, 4>, (const void *)k_dphase<float, 4>,
                        (const void *)k_dphase<double, 8>, (const void *)k_dphase<float, 8>};
   for (const void *f : fns)

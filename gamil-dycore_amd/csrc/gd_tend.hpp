@@ -42,7 +42,7 @@ constexpr int TC_DU = 1, TC_DV = 2, TC_DGD = 4, TC_RING = 8, TC_ALL = 15;
 // issued without a condition: past N a lane reads the ghost rows (gd 1,
 // V 0), whose term (1 + 1) * 0 = +0 leaves the sum as it is.
 __device__ __forceinline__ double ring_part(const Args &a, const Member &mb, int j) {
-  const int N = a.N, M = a.M, P = a.pitch, tid = threadIdx.x;
+  const int N = a.N, M = a.M, P = a.pitch, tid = ltid();
   const double *gd = mb.st(a.s_in, FGD);
   const double *Vg = mb.st(a.s_in, FV);
   const double *one = a.ghost, *zero = a.ghost + P;
@@ -82,7 +82,7 @@ struct SegLoads {
 
 template <int PASS, bool IN_FULL>
 __device__ __forceinline__ void seg_load(const Args &a, const Member &mb, int j, int i0, int W, SegLoads &L) {
-  const int N = a.N, M = a.M, P = a.pitch, tid = threadIdx.x;
+  const int N = a.N, M = a.M, P = a.pitch, tid = ltid();
   const double *gd = mb.st(a.s_in, FGD);
   const double *Ug = mb.st(a.s_in, FU);
   const double *Vg = mb.st(a.s_in, FV);
@@ -167,7 +167,7 @@ __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, 
   double(*sV)[kLW] = (double(*)[kLW])(smem + 3 * kLW * sizeof(double));  // V half rows j-1, j
   CT(*su)[kLW] = (CT(*)[kLW])(smem + 5 * kLW * sizeof(double));          // u rows j, j+1
   CT(*sv)[kLW] = (CT(*)[kLW])(smem + 5 * kLW * sizeof(double) + 2 * kLW * sizeof(CT));  // v j-1, j
-  const int tid = threadIdx.x;
+  const int tid = ltid();
   constexpr bool DU = (CM & TC_DU) != 0, DV = (CM & TC_DV) != 0, DG = (CM & TC_DGD) != 0;
   const bool has_u = j >= 1 && j <= M - 2;
   const bool has_v = j <= M - 2;
@@ -353,7 +353,7 @@ template <typename CT, int PASS, bool IN_FULL, int CM = TC_ALL>
 __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &mb, const RowCoef &cj, int j,
                                                   int i0, int W, bool stg_u, bool stg_v, bool stg_g,
                                                   unsigned char *smem, double *red) {
-  const int M = a.M, P = a.pitch, tid = threadIdx.x;
+  const int M = a.M, P = a.pitch, tid = ltid();
   // pole mass flux sums over the whole ring (dycore_mod.F90:572-596); every
   // segment of a pole row forms the same sum in the same order (ring_part,
   // as pole_flux).  Pole tiles finish these loads before issuing the halo
@@ -390,7 +390,7 @@ __device__ __forceinline__ int far_row(const Args &a, int r) {
 
 template <typename CT, int PASS, bool IN_FULL, bool FUSED>
 __device__ __forceinline__ void tend_tile(const Args &a, int t, unsigned char *smem, double *red) {
-  const int M = a.M, P = a.pitch, nseg = a.nseg, tid = threadIdx.x;
+  const int M = a.M, P = a.pitch, nseg = a.nseg, tid = ltid();
   const int nrows = FUSED ? a.nfar_rows : M;
   const int mloc = t / (nrows * nseg), rem = t - mloc * (nrows * nseg);
   const int r = rem / nseg, g = rem - (rem / nseg) * nseg;

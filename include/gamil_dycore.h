@@ -388,26 +388,32 @@ int64_t gd_device_bytes(const gd_handle *h);
  * measurement hook. */
 enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2, GD_TIME_PHASE = 3, GD_TIME_FILTER_MM = 4 };
 
-/* 1 when the handle's step, at its current active-member count, runs each
- * tendency phase as one fused launch, else 0.  By default phases are fused
- * with filtered rows while active members x num_lat x ceil(num_lon/254) row
- * tiles stay within 24576 (num_lon <= 1024) or 6144 (num_lon <= 2048), and
- * gd_autotune may choose either form; GD_FUSED=0 or 1 in the environment
- * when the handle is created forces separate or fused launches.  No
+/* How the handle's step, at its current active-member count, launches each
+ * tendency phase: 0 a tendency launch then a filter launch, 1 one fused
+ * launch (tendency, filter rows and epilogue), 2 one fused launch with a
+ * workgroup per latitude row (rows of 255 to 508 longitudes).  By default
+ * phases are fused with filtered rows while active members x num_lat x
+ * ceil(num_lon/254) row tiles stay within 24576 (num_lon <= 1024) or 6144
+ * (num_lon <= 2048), in form 2 while its workgroups fit two per compute
+ * unit; gd_autotune may choose any form.  GD_FUSED=0, 1 or 2 in the
+ * environment when the handle is created forces a form (2 falls back to 1
+ * where rows are not of two segments).  Every form gives the same bits.  No
  * counterpart in the reference. */
 int gd_fused_phases(const gd_handle *h);
 
 /* Tile autotuner (no counterpart in the reference).  For the current
  * active-member count, times `steps` steps (1..512) of every launch shape
- * that cannot change a result -- phases fused or as separate tendency and
- * filter launches, and the workgroups per member of the corrector update --
+ * that cannot change a result -- the form of the phase launches (separate
+ * tendency and filter launches, fused, fused with a workgroup per row; see
+ * gd_fused_phases), and the workgroups per member of the corrector update --
  * from the members' current state, puts that state back, and keeps the
  * fastest shape for later gd_step calls at this active-member count (the
  * results are bit for bit those of the default shapes).  cache_path (may be
  * NULL) is a text file of earlier results keyed by the device architecture,
  * grid, member counts, precision and schemes: a cached entry is applied
  * without timing, a new result is appended.  Outputs (may be NULL): the
- * chosen fused flag, workgroups per member and milliseconds per step. */
+ * chosen form (0, 1 or 2), workgroups per member and milliseconds per
+ * step. */
 int gd_autotune(gd_handle *h, int32_t steps, const char *cache_path, int32_t *fused,
                 int32_t *upd_wpm, double *ms_per_step);
 int gd_time_kernel(gd_handle *h, int32_t kernel, int32_t reps, double *us_per_launch);

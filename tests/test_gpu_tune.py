@@ -39,7 +39,8 @@ def _run(gd, N, M, dt, steps, members, prec, tune, active=None):
 
 @pytest.mark.parametrize("N,M,dt,members,active,prec", [(72, 37, 900.0, 3, None, "mixed"),
                                                       (72, 37, 900.0, 4, 3, "fp64"),
-                                                      (360, 181, 240.0, 8, None, "mixed")])
+                                                      (360, 181, 240.0, 8, None, "mixed"),
+                                                      (360, 181, 240.0, 1, None, "fp64")])
 def test_tuned_run_gives_default_bits(gd, N, M, dt, members, active, prec):
     print(f"autotune {N}x{M} x{members} active={active} {prec}", flush=True)
     a = _run(gd, N, M, dt, 4, members, prec, False, active)
