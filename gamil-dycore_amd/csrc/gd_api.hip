@@ -239,17 +239,19 @@ constexpr int64_t kFusedMaxTiles = 24576;
 constexpr int64_t kFusedMaxTilesWide = 6144;
 
 // Form of the phase launches: 0 k_tend + k_filter, 1 k_phase, 2 k_wphase.
-// k_wphase (rows of two segments) by default while its workgroups -- one per
-// filter row and one per far row -- fit two per CU: measured on MI355X
-// (mixed, one or two runs of 288x145, 360x181, 480x241) 5-10% faster than
-// k_phase, and 4-20% slower once they take a second round.
+// k_wphase for rows of two segments by default while its workgroups -- one
+// per filter row and one per far row -- fit two per CU: measured on MI355X
+// (mixed, one or two runs of 288x145, 360x181, 480x241) 3-10% faster than
+// k_phase, and 4-20% slower once they take a second round.  Rows of three
+// segments (one run of 720x361: 0.83-0.85 of k_phase) take it only when
+// asked for or tuned.
 int phase_form(const gd_handle *h) {
   if (!h->fused_ok || h->fused_mode == 0) return 0;
   if (h->fused_mode == 1) return 1;
   if (h->fused_mode == 2) return h->wide_ok ? 2 : 1;
   auto it = h->tuned_fused.find(h->nact);
   if (it != h->tuned_fused.end()) return it->second == 2 && !h->wide_ok ? 1 : it->second;
-  if (h->wide_ok) {
+  if (h->wide_ok && (h->N + kSeg - 1) / kSeg == 2) {
     const int64_t wgs = (int64_t)h->nact * ((int64_t)(h->frows.size() / h->nmem) + h->nfar_rows);
     if (wgs <= 2 * (int64_t)h->ncu) return 2;
   }
@@ -729,8 +731,8 @@ int gd_create(const gd_config *cfg, int32_t n_members, gd_handle **out) {
                   phase_lds_bytes(h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
     const int nseg = (h->N + kSeg - 1) / kSeg;
     h->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    h->wide_ok = h->fused_ok && nseg == 2 &&
-                 wphase_lds_bytes(nseg, h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
+    h->wide_ok = h->fused_ok && (nseg == 2 || nseg == 3) &&
+                 wphase_lds_bytes(h->N, h->fft_L, h->prec) <= prop.sharedMemPerBlock;
   }
   if (h->filt_mm && (h->N < 64 || h->N > 1440 || filter_mm_lds_bytes(h->N, h->filt_mm == 2) > prop.sharedMemPerBlock)) {
     delete h;

@@ -498,41 +498,55 @@ __device__ __forceinline__ void head_sum_n(double (&v)[K], double *red) {
 }
 
 // filter_row<CT, KP, true> (the row already in b0, a rescaled phase) for a
-// workgroup of NT threads with one longitude each (N <= NT): the loads, the
-// transforms' butterflies and the epilogue spread over all NT threads.  The
-// sums keep filter_row's order: thread t < kBlock adds the products of
-// longitudes t, t + kBlock, ... in turn (staged through scr, 2 NT doubles),
-// then head_sum_n reduces those kBlock partial sums as block_sum_n does, so
-// both give the same bits.  Ends with a barrier.
-template <typename CT, int NT>
+// workgroup of NT threads with KW longitudes each (l = thread + k NT, N <=
+// KW NT): the loads, the transforms' butterflies and the epilogue spread
+// over all NT threads.  The sums keep filter_row's order: thread t < kBlock
+// adds the products of longitudes t, t + kBlock, ... in turn (staged through
+// scr, 2 N doubles), then head_sum_n reduces those kBlock partial sums as
+// block_sum_n does, so both give the same bits.  Ends with a barrier.
+template <typename CT, int NT, int KW>
 __device__ __forceinline__ void filter_row_w(const Args &a, const Member &mb, const FiltRow &fr, double2 *tw,
                                              double2 *b0, double2 *b1, double *scr, double *red, bool zero) {
-  const int N = a.N, P = a.pitch, L = a.fft_L, l = threadIdx.x;
+  const int N = a.N, P = a.pitch, L = a.fft_L, tid = threadIdx.x;
   const double sc = 1.0 / (double)L;
   const int comp = fr.comp;
   const size_t o = (size_t)fr.row * P;
   const int ystep = a.fft_half ? 1 : 2;
-  const bool on = l < N;
   const double *wp = mb.st(a.s_in, comp == CDU ? FU : (comp == CDV ? FV : FGD)) + o;
   const bool gdrow = comp == CDGD;
-  double w = 0.0, wg = 0.0, pre = 0.0;
-  if (on) {
-    if (!zero) {
-      w = wp[l];
-      if (gdrow) wg = mb.ghs()[o + l];
+  double x[KW], w[KW], wg[KW], pre[KW], yv[KW];
+#pragma unroll
+  for (int k = 0; k < KW; ++k) {
+    const int l = tid + k * NT;
+    double wv = 0.0, gv = 0.0, pv = 0.0;
+    if (l < N) {
+      if (!zero) {
+        wv = wp[l];
+        if (gdrow) gv = mb.ghs()[o + l];
+      }
+      pv = phase_pre<CT>(a, mb, comp, o + l);
     }
-    pre = phase_pre<CT>(a, mb, comp, o + l);
+    w[k] = wv;
+    wg[k] = gv;
+    pre[k] = pv;
+    yv[k] = 0.0;
   }
   // the row values are read before the first pass; b0 is first written by
   // the second pass, after a barrier
-  const double x = on && !zero ? ((const double *)b0)[(size_t)l * ystep] : 0.0;
+#pragma unroll
+  for (int k = 0; k < KW; ++k) {
+    const int l = tid + k * NT;
+    x[k] = l < N && !zero ? ((const double *)b0)[(size_t)l * ystep] : 0.0;
+  }
   const int nk = (N + kBlock - 1) / kBlock;  // longitudes per head thread
-  double yv = 0.0;
   double s12[2] = {0.0, 0.0};
   if (!zero) {
     const Tw twt{tw, N};
     int cur = fft_any<4, NT>(b0, b1, 0, a, twt, false);
-    if (gdrow) w = w + wg;
+    if (gdrow) {
+#pragma unroll
+      for (int k = 0; k < KW; ++k) w[k] = w[k] + wg[k];
+    }
     if (a.fft_half)
       real_mid<NT>(cur ? b1 : b0, cur ? b0 : b1, L, fr.cut, twt, sc);
     else
@@ -540,15 +554,19 @@ __device__ __forceinline__ void filter_row_w(const Args &a, const Member &mb, co
     __syncthreads();
     cur = fft_any<4, NT>(b0, b1, cur ^ 1, a, twt, true);
     const double *y = (const double *)(cur ? b1 : b0);
-    yv = on ? y[(size_t)l * ystep] : 0.0;
-    if (on) {
-      scr[2 * l] = x * w;
-      scr[2 * l + 1] = yv * w;
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+      const int l = tid + k * NT;
+      if (l < N) {
+        yv[k] = y[(size_t)l * ystep];
+        scr[2 * l] = x[k] * w[k];
+        scr[2 * l + 1] = yv[k] * w[k];
+      }
     }
     __syncthreads();
-    if (l < kBlock)
+    if (tid < kBlock)
       for (int k = 0; k < nk; ++k) {
-        const int e = l + k * kBlock;
+        const int e = tid + k * kBlock;
         if (e < N) {
           s12[0] += scr[2 * e];
           s12[1] += scr[2 * e + 1];
@@ -559,22 +577,31 @@ __device__ __forceinline__ void filter_row_w(const Args &a, const Member &mb, co
   const double s1 = s12[0], s2 = s12[1];
   // rows with |s1| <= filter_inner_product_threshold stay unfiltered
   const bool filt = fabs(s1) > 1.0e-16;
-  double p1 = 0.0, p2 = 0.0;
-  if (on) {
-    const double T = filt ? yv * s1 / s2 : x;
-    phase_store<CT>(a, mb, comp, o + l, T, pre, fr.wrow, p1, p2);
+  double pk[KW][2];
+#pragma unroll
+  for (int k = 0; k < KW; ++k) {
+    const int l = tid + k * NT;
+    pk[k][0] = pk[k][1] = 0.0;
+    if (l < N) {
+      const double T = filt ? yv[k] * s1 / s2 : x[k];
+      phase_store<CT>(a, mb, comp, o + l, T, pre[k], fr.wrow, pk[k][0], pk[k][1]);
+    }
   }
   if (a.phase == PH_C) {
     // (scr was last read before head_sum_n's barriers)
-    if (on) {
-      scr[2 * l] = p1;
-      scr[2 * l + 1] = p2;
+#pragma unroll
+    for (int k = 0; k < KW; ++k) {
+      const int l = tid + k * NT;
+      if (l < N) {
+        scr[2 * l] = pk[k][0];
+        scr[2 * l + 1] = pk[k][1];
+      }
     }
     __syncthreads();
     double p[2] = {0.0, 0.0};
-    if (l < kBlock)
+    if (tid < kBlock)
       for (int k = 0; k < nk; ++k) {
-        const int e = l + k * kBlock;
+        const int e = tid + k * kBlock;
         if (e < N) {
           p[0] += scr[2 * e];
           p[1] += scr[2 * e + 1];
@@ -584,12 +611,12 @@ __device__ __forceinline__ void filter_row_w(const Args &a, const Member &mb, co
     // the whole row's products go to segment 0 of its partials, the other
     // segments of the row hold zero
     double *pp = mb.part() + ((size_t)(fr.row * 3 + comp) * a.nseg) * 2;
-    if (l == 0) {
+    if (tid == 0) {
       pp[0] = p[0];
       pp[1] = p[1];
-    } else if (l < a.nseg) {
-      pp[2 * l] = 0.0;
-      pp[2 * l + 1] = 0.0;
+    } else if (tid < a.nseg) {
+      pp[2 * tid] = 0.0;
+      pp[2 * tid + 1] = 0.0;
     }
   }
   __syncthreads();  // LDS free for the caller

@@ -45,10 +45,10 @@ hipError_t launch_phase(const Args &a, int prec, int pass, bool in_full, hipStre
 // (gd_phase.hip); N <= 4 * kBlock
 hipError_t launch_dphase(const Args &a, int prec, hipStream_t s);
 size_t phase_lds_bytes(int N, int L, int prec);
-// the same phase with one workgroup per row (k_wphase, gd_phase.hip): rows of
-// two segments; a.ntiles counts segment tiles as for launch_phase
+// the same phase with workgroups of two tiles (k_wphase, gd_phase.hip): rows
+// of two or three segments; a.ntiles counts segment tiles as for launch_phase
 hipError_t launch_wphase(const Args &a, int prec, int pass, bool in_full, hipStream_t s);
-size_t wphase_lds_bytes(int nseg, int N, int L, int prec);
+size_t wphase_lds_bytes(int N, int L, int prec);
 // momentum rows of the filter as batched MFMA products (gd_filter_mm.hip):
 // one workgroup per (momentum slot in a.mslots, block of 16 members)
 hipError_t launch_filter_mm(const Args &a, bool bf16, hipStream_t s);

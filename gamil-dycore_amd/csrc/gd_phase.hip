@@ -153,25 +153,30 @@ k_phase(const Args a) {
 }
 
 // ---------------------------------------------------------------------------
-// k_wphase: k_phase with one workgroup per row.  A workgroup holds S tiles
-// of kBlock threads side by side, S = the row's segment count.  Its filter
-// tiles evaluate the S segments of their row at once (one per tile, each
-// with its own stencil rows) instead of one after another, and filter the
-// row with one longitude per thread (filter_row_w); its row tiles are the S
-// segment tiles of one far row, which take the same barriers.  Tile t's
-// arithmetic is that of k_phase, so both launches give the same bits.
+// k_wphase: k_phase with workgroups of two tiles (kWTiles x kBlock threads).
+// A filter workgroup evaluates its row's segments two at a time, one per
+// tile with its own stencil rows, instead of one after another, and filters
+// the row with one or two longitudes per thread (filter_row_w).  A row
+// workgroup runs two consecutive row tiles (t = 2 r, 2 r + 1), of one row or
+// of two; when one of them is a pole-row tile the other takes the same two
+// barriers (seg_eval's bar2), and a spare tile past the last one does the
+// last tile's work without its stores.  Tile t's arithmetic is that of
+// k_phase, so both launches give the same bits.
 //
-// Launched for rows of two segments (255 to 508 longitudes), where a
-// workgroup is eight waves and two fit a CU at 96 VGPRs: a 360x181 run steps
-// 10% faster than with k_phase.  Rows of three segments measured slower (a
-// 720x361 run at 0.80 to 0.85 of k_phase): one twelve-wave workgroup per CU
-// at 96 VGPRs leaves the row workgroups a second round, and two per CU
-// (80 VGPRs) spill.
+// Rows of two segments (255 to 508 longitudes) map one row to a workgroup.
+// Measured on MI355X against k_phase: single runs of 288x145, 360x181 and
+// 480x241 step 9-10% faster, two runs of 360x181 3-5%.  At 720x361 (three
+// segments) a single run steps at 0.83-0.85 of k_phase: its 633 workgroups
+// take a second round at two per CU, where k_phase's 1,087 tiles fit five
+// per CU.  Workgroups of three tiles (one per segment) were no better: one
+// twelve-wave workgroup per CU at 96 VGPRs, and two per CU (80 VGPRs) spill.
 //
-// LDS: S stencil regions of wphase_region bytes (region 0 later holds FFT
-// buffer b1, region 1 the filter's partial products), then the row buffer
-// b0 and the twiddle table.
+// LDS: kWTiles stencil regions of wphase_region bytes (region 0 later holds
+// FFT buffer b1, region 1 the filter's products), then the row buffer b0
+// and the twiddle table.
 // ---------------------------------------------------------------------------
+
+constexpr int kWTiles = 2;
 
 template <typename CT>
 __host__ __device__ inline size_t wphase_region(int L) {
@@ -180,77 +185,102 @@ __host__ __device__ inline size_t wphase_region(int L) {
 }
 
 template <typename CT>
-__host__ __device__ inline size_t wphase_lds(int S, int N, int L) {
-  return (size_t)S * wphase_region<CT>(L) + (size_t)L * 16 + (size_t)tw_padded(N) * 16;
+__host__ __device__ inline size_t wphase_lds(int N, int L) {
+  return (size_t)kWTiles * wphase_region<CT>(L) + (size_t)L * 16 + (size_t)tw_padded(N) * 16;
 }
 
-// Segment s of component CM of row j into the row buffer: tile s of the
-// workgroup, with the arithmetic of near_row's segment s.
+// Component CM of row j into the row buffer: tile s of the workgroup takes
+// segments s, s + kWTiles, ... with the arithmetic of near_row (the next
+// segment's loads issued before the current one is evaluated).  A tile with
+// no segment left in a round evaluates an empty one (W = 0: its loads stay
+// inside the row, nothing is written) to take the round's barriers.
 template <typename CT, int PASS, bool IN_FULL, int CM>
-__device__ __forceinline__ void wnear_seg(const Args &a, const Member &mb, const RowCoef &cj, int j, int s,
+__device__ __forceinline__ void wnear_row(const Args &a, const Member &mb, const RowCoef &cj, int j, int s,
                                           double *rowbuf, unsigned char *st, double *red) {
-  const int N = a.N, tid = ltid();
+  const int N = a.N, nseg = a.nseg, tid = ltid();
+  const int rounds = (nseg + kWTiles - 1) / kWTiles;
   const RowNbr nb = row_nbr(a, j);
-  const int i0 = s * kSeg, W = min(N - i0, kSeg);
-  SegLoads ld;
-  seg_load<PASS, IN_FULL>(a, mb, j, i0, W, ld);
-  const TendPoint pt = seg_eval<CT, PASS, IN_FULL, CM>(a, cj, nb, j, i0, W, ld, false, 0.0,
-                                                       TendPoint{0, 0, 0, 0, 0, 0}, st, red);
-  if (tid < W) row_put(a, rowbuf, i0 + tid, CM == TC_DU ? pt.du : (CM == TC_DV ? pt.dv : pt.dgd));
+  auto seg_i0 = [&](int g) { return g < nseg ? g * kSeg : 0; };
+  auto seg_w = [&](int g) { return g < nseg ? min(N - g * kSeg, kSeg) : 0; };
+  SegLoads cur;
+  seg_load<PASS, IN_FULL>(a, mb, j, seg_i0(s), seg_w(s), cur);
+  for (int k = 0; k < rounds; ++k) {
+    const int g = k * kWTiles + s, gn = g + kWTiles;
+    SegLoads nxt;
+    if (k + 1 < rounds) seg_load<PASS, IN_FULL>(a, mb, j, seg_i0(gn), seg_w(gn), nxt);
+    if (k) __syncthreads();  // the previous round's stencil rows have been read
+    const int i0 = seg_i0(g), W = seg_w(g);
+    const TendPoint pt = seg_eval<CT, PASS, IN_FULL, CM>(a, cj, nb, j, i0, W, cur, false, 0.0,
+                                                         TendPoint{0, 0, 0, 0, 0, 0}, st, red);
+    if (tid < W) row_put(a, rowbuf, i0 + tid, CM == TC_DU ? pt.du : (CM == TC_DV ? pt.dv : pt.dgd));
+    cur = nxt;
+  }
 }
 
-template <typename CT, int PASS, bool IN_FULL, int S>
+template <typename CT, int PASS, bool IN_FULL, int KW>
 __device__ __forceinline__ void wnear_tile(const Args &a, const FiltRow fr, unsigned char *smem, double *red) {
-  constexpr int NT = S * kBlock;
+  constexpr int NT = kWTiles * kBlock;
   const int N = a.N, L = a.fft_L, s = threadIdx.x / kBlock;
   const Member mb = member_at(a, fr.member);
   const int j = fr.row;
   const RowCoef cj = a.coef[j];
   const size_t R = wphase_region<CT>(L);
   double2 *b1 = (double2 *)smem;
-  double2 *b0 = (double2 *)(smem + S * R);
+  double2 *b0 = (double2 *)(smem + kWTiles * R);
   double2 *tw = b0 + L;
   tw_to_lds<NT>(a, tw);
   double *rowbuf = (double *)b0;
   unsigned char *st = smem + s * R;
   double *rd = red + s * 4 * 6;
   if (fr.comp == CDU) {
-    wnear_seg<CT, PASS, IN_FULL, TC_DU>(a, mb, cj, j, s, rowbuf, st, rd);
+    wnear_row<CT, PASS, IN_FULL, TC_DU>(a, mb, cj, j, s, rowbuf, st, rd);
   } else if (fr.comp == CDV) {
-    wnear_seg<CT, PASS, IN_FULL, TC_DV>(a, mb, cj, j, s, rowbuf, st, rd);
+    wnear_row<CT, PASS, IN_FULL, TC_DV>(a, mb, cj, j, s, rowbuf, st, rd);
   } else if (PASS != 2 && (j == 0 || j == a.M - 1)) {
     // a pole row's dgd is minus the pole flux at every longitude; each tile
     // forms the same ring sum (dycore_mod.F90:572-596)
     const double pole = pole_flux(a, mb, j, rd);
     for (int l = threadIdx.x; l < N; l += NT) row_put(a, rowbuf, l, -pole);
   } else if (PASS != 2) {
-    wnear_seg<CT, PASS, IN_FULL, TC_DGD>(a, mb, cj, j, s, rowbuf, st, rd);
+    wnear_row<CT, PASS, IN_FULL, TC_DGD>(a, mb, cj, j, s, rowbuf, st, rd);
   }
   // (the slow pass has no mass continuity term: its dgd rows are zero rows,
   // which filter_row_w leaves unfiltered without reading b0)
   __builtin_amdgcn_s_waitcnt(0);  // the twiddle copy has landed
   __syncthreads();  // row and twiddles published; the stencil rows are free
-  filter_row_w<CT, NT>(a, mb, fr, tw, b0, b1, (double *)(smem + R), red, PASS == 2 && fr.comp == CDGD);
+  filter_row_w<CT, NT, KW>(a, mb, fr, tw, b0, b1, (double *)(smem + R), red, PASS == 2 && fr.comp == CDGD);
 }
 
-// Two workgroups of two tiles per CU (16 of the 20 waves that 96 VGPRs allow).
-template <typename CT, int PASS, bool IN_FULL, int S>
-__global__ void __launch_bounds__(S * kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+// Two workgroups per CU (16 of the 20 waves that 96 VGPRs allow).  KW:
+// longitudes per thread in the filter (1 up to 512 longitudes, 2 up to 1024).
+template <typename CT, int PASS, bool IN_FULL, int KW>
+__global__ void __launch_bounds__(kWTiles * kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_wphase(const Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[S * 4 * 6];
+  __shared__ double red[kWTiles * 4 * 6];
   const int b = blockIdx.x;
   if (b < a.nnear_pad) {
     const int t = (b % kXcd) * (a.nnear_pad / kXcd) + b / kXcd;
-    if (t < a.nnear) wnear_tile<CT, PASS, IN_FULL, S>(a, a.frows[t], smem, red);
+    if (t < a.nnear) wnear_tile<CT, PASS, IN_FULL, KW>(a, a.frows[t], smem, red);
     return;
   }
-  // row workgroups: (member, far row), each XCD owning a contiguous band
+  // row workgroups: tiles 2 r and 2 r + 1, each XCD owning a contiguous band
   const int bb = b - a.nnear_pad, per = (gridDim.x - a.nnear_pad) / kXcd;
   const int r = (bb % kXcd) * per + bb / kXcd;
-  if (r * S >= a.ntiles) return;
+  const int t0 = r * kWTiles;
+  if (t0 >= a.ntiles) return;
+  bool bar2 = false;
+  if (PASS != 2) {
+#pragma unroll
+    for (int q = 0; q < kWTiles; ++q) {
+      const int jq = far_tile_row(a, min(t0 + q, a.ntiles - 1));
+      bar2 = bar2 || jq == 0 || jq == a.M - 1;
+    }
+  }
   const int s = threadIdx.x / kBlock;
-  tend_tile<CT, PASS, IN_FULL, true>(a, r * S + s, smem + s * wphase_region<CT>(a.fft_L), red + s * 4 * 6);
+  const bool live = t0 + s < a.ntiles;
+  tend_tile<CT, PASS, IN_FULL, true>(a, live ? t0 + s : a.ntiles - 1, smem + s * wphase_region<CT>(a.fft_L),
+                                     red + s * 4 * 6, bar2, live);
 }
 
 // ---------------------------------------------------------------------------
@@ -366,13 +396,13 @@ hipError_t launch_phase(const Args &a, int prec, int pass, bool in_full, hipStre
   return prec == 0 ? launch_phase_k<double, 4>(a, pass, in_full, s) : launch_phase_k<float, 4>(a, pass, in_full, s);
 }
 
-template <typename CT, int S>
+template <typename CT, int KW>
 static hipError_t launch_wphase_k(const Args &a, int pass, bool in_full, hipStream_t s) {
-  const int nrow = a.ntiles / S;  // (member, far row) workgroups
-  const int rows = ((nrow + kXcd - 1) / kXcd) * kXcd;
-  dim3 g(a.nnear_pad + rows), b(S * kBlock);
-  const size_t lds = wphase_lds<CT>(S, a.N, a.fft_L);
-#define GD_PH(P, F) hipLaunchKernelGGL((k_wphase<CT, P, F, S>), g, b, lds, s, a)
+  const int nwg = (a.ntiles + kWTiles - 1) / kWTiles;  // row workgroups
+  const int rows = ((nwg + kXcd - 1) / kXcd) * kXcd;
+  dim3 g(a.nnear_pad + rows), b(kWTiles * kBlock);
+  const size_t lds = wphase_lds<CT>(a.N, a.fft_L);
+#define GD_PH(P, F) hipLaunchKernelGGL((k_wphase<CT, P, F, KW>), g, b, lds, s, a)
   if (pass == 0) { if (in_full) GD_PH(0, true); else GD_PH(0, false); }
   else if (pass == 1) { if (in_full) GD_PH(1, true); else GD_PH(1, false); }
   else { if (in_full) GD_PH(2, true); else GD_PH(2, false); }
@@ -381,20 +411,22 @@ static hipError_t launch_wphase_k(const Args &a, int pass, bool in_full, hipStre
 }
 
 hipError_t launch_wphase(const Args &a, int prec, int pass, bool in_full, hipStream_t s) {
-  // one workgroup per row of two segments, every segment a tile
-  if (a.nseg != 2 || a.N > a.nseg * kBlock || a.ntiles % a.nseg != 0) return hipErrorInvalidValue;
+  // rows of two or three segments: one or two longitudes per filter thread
+  if (a.nseg < 2 || a.nseg > 3 || a.N > a.nseg * kSeg) return hipErrorInvalidValue;
+  if (a.nseg == 2)
+    return prec == 0 ? launch_wphase_k<double, 1>(a, pass, in_full, s) : launch_wphase_k<float, 1>(a, pass, in_full, s);
   return prec == 0 ? launch_wphase_k<double, 2>(a, pass, in_full, s) : launch_wphase_k<float, 2>(a, pass, in_full, s);
 }
 
-size_t wphase_lds_bytes(int nseg, int N, int L, int prec) {
-  return prec == 0 ? wphase_lds<double>(nseg, N, L) : wphase_lds<float>(nseg, N, L);
+size_t wphase_lds_bytes(int N, int L, int prec) {
+  return prec == 0 ? wphase_lds<double>(N, L) : wphase_lds<float>(N, L);
 }
 
-template <typename CT, int S>
+template <typename CT, int KW>
 static hipError_t configure_wphase_k(int max_lds) {
-  const void *fns[] = {(const void *)k_wphase<CT, 0, true, S>,  (const void *)k_wphase<CT, 0, false, S>,
-                       (const void *)k_wphase<CT, 1, true, S>,  (const void *)k_wphase<CT, 1, false, S>,
-                       (const void *)k_wphase<CT, 2, true, S>,  (const void *)k_wphase<CT, 2, false, S>};
+  const void *fns[] = {(const void *)k_wphase<CT, 0, true, KW>,  (const void *)k_wphase<CT, 0, false, KW>,
+                       (const void *)k_wphase<CT, 1, true, KW>,  (const void *)k_wphase<CT, 1, false, KW>,
+                       (const void *)k_wphase<CT, 2, true, KW>,  (const void *)k_wphase<CT, 2, false, KW>};
   for (const void *f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
     if (e != hipSuccess) return e;
@@ -420,6 +452,8 @@ hipError_t configure_phase(int max_lds) {
   if ((e = configure_phase_k<float, 4>(max_lds)) != hipSuccess) return e;
   if ((e = configure_phase_k<double, 8>(max_lds)) != hipSuccess) return e;
   if ((e = configure_phase_k<float, 8>(max_lds)) != hipSuccess) return e;
+  if ((e = configure_wphase_k<double, 1>(max_lds)) != hipSuccess) return e;
+  if ((e = configure_wphase_k<float, 1>(max_lds)) != hipSuccess) return e;
   if ((e = configure_wphase_k<double, 2>(max_lds)) != hipSuccess) return e;
   if ((e = configure_wphase_k<float, 2>(max_lds)) != hipSuccess) return e;
   const void *fns[] = {(const void *)k_dphase<double, 4>, (const void *)k_dphase<float, 4>,

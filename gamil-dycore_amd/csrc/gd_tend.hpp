@@ -156,11 +156,14 @@ __device__ __forceinline__ RowNbr row_nbr(const Args &a, int j) {
 // The rest of a segment once its loads are issued: the LDS rows, one
 // barrier (or the pole ring sum pacc's block reduction when pole_row) and
 // the components CM of the own point, into out.  The LDS rows only other
-// components read are not written, so their loads drop out.
+// components read are not written, so their loads drop out.  bar2: a tile
+// sharing its workgroup with a pole-row tile takes two barriers as that
+// tile's reduction does, so every tile of the workgroup takes the same
+// barriers.
 template <typename CT, int PASS, bool IN_FULL, int CM>
 __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, const RowNbr &nb, int j, int i0,
                                               int W, const SegLoads &L, bool pole_row, double pacc, TendPoint out,
-                                              unsigned char *smem, double *red) {
+                                              unsigned char *smem, double *red, bool bar2 = false) {
   const int N = a.N, M = a.M;
   double *sG1 = (double *)smem;                          // G row j
   double(*sU)[kLW] = (double(*)[kLW])(smem + 1 * kLW * sizeof(double));  // U rows j, j+1
@@ -238,6 +241,7 @@ __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, 
     pole = sum * 2.0 / N / kRadius / a.dlat;
   } else {
     __syncthreads();
+    if (bar2) __syncthreads();
   }
   if (!own) return out;
 
@@ -352,7 +356,7 @@ __device__ __forceinline__ TendPoint seg_eval(const Args &a, const RowCoef &cj, 
 template <typename CT, int PASS, bool IN_FULL, int CM = TC_ALL>
 __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &mb, const RowCoef &cj, int j,
                                                   int i0, int W, bool stg_u, bool stg_v, bool stg_g,
-                                                  unsigned char *smem, double *red) {
+                                                  unsigned char *smem, double *red, bool bar2 = false) {
   const int M = a.M, P = a.pitch, tid = ltid();
   // pole mass flux sums over the whole ring (dycore_mod.F90:572-596); every
   // segment of a pole row forms the same sum in the same order (ring_part,
@@ -371,7 +375,7 @@ __device__ __forceinline__ TendPoint tend_segment(const Args &a, const Member &m
     if (!stg_v && j <= M - 2) out.pre_v = phase_pre<CT>(a, mb, CDV, o);
     if (!stg_g) out.pre_g = phase_pre<CT>(a, mb, CDGD, o);
   }
-  return seg_eval<CT, PASS, IN_FULL, CM>(a, cj, nb, j, i0, W, L, pole_row, pacc, out, smem, red);
+  return seg_eval<CT, PASS, IN_FULL, CM>(a, cj, nb, j, i0, W, L, pole_row, pacc, out, smem, red, bar2);
 }
 
 // Row tile t = (member, row, segment of kSeg longitudes) of a phase launch.
@@ -388,8 +392,17 @@ __device__ __forceinline__ int far_row(const Args &a, int r) {
   return 0;
 }
 
+// The row of fused-launch row tile t.
+__device__ __forceinline__ int far_tile_row(const Args &a, int t) {
+  const int per = a.nfar_rows * a.nseg;
+  return far_row(a, (t - (t / per) * per) / a.nseg);
+}
+
+// bar2: see seg_eval.  live = false: the tile's work and barriers without
+// its stores (a workgroup's spare tile past the last one).
 template <typename CT, int PASS, bool IN_FULL, bool FUSED>
-__device__ __forceinline__ void tend_tile(const Args &a, int t, unsigned char *smem, double *red) {
+__device__ __forceinline__ void tend_tile(const Args &a, int t, unsigned char *smem, double *red, bool bar2 = false,
+                                          bool live = true) {
   const int M = a.M, P = a.pitch, nseg = a.nseg, tid = ltid();
   const int nrows = FUSED ? a.nfar_rows : M;
   const int mloc = t / (nrows * nseg), rem = t - mloc * (nrows * nseg);
@@ -403,9 +416,9 @@ __device__ __forceinline__ void tend_tile(const Args &a, int t, unsigned char *s
   const int st_g = cj.stage[CDGD];
   const int i0 = g * kSeg, W = min(a.N - i0, kSeg);
   const TendPoint pt = tend_segment<CT, PASS, IN_FULL>(a, mb, cj, j, i0, W, st_u >= 0, st_v >= 0, st_g >= 0,
-                                                       smem, red);
+                                                       smem, red, bar2);
   double p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
-  if (tid < W) {
+  if (live && tid < W) {
     const int i = i0 + tid;
     const size_t o = (size_t)j * P + i;
     // (pole rows: du = 0 goes through the epilogue like any other value)
@@ -427,7 +440,7 @@ __device__ __forceinline__ void tend_tile(const Args &a, int t, unsigned char *s
   if (a.phase == PH_C) {
     double pq[6] = {p1[0], p2[0], p1[1], p2[1], p1[2], p2[2]};
     block_sum_n<6>(pq, red);
-    if (tid == 0) {
+    if (live && tid == 0) {
       const int st[3] = {st_u, st_v, st_g};
 #pragma unroll
       for (int c = 0; c < 3; ++c) {

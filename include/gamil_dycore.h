@@ -390,15 +390,16 @@ enum { GD_TIME_TEND = 0, GD_TIME_FILTER = 1, GD_TIME_UPDATE = 2, GD_TIME_PHASE =
 
 /* How the handle's step, at its current active-member count, launches each
  * tendency phase: 0 a tendency launch then a filter launch, 1 one fused
- * launch (tendency, filter rows and epilogue), 2 one fused launch with a
- * workgroup per latitude row (rows of 255 to 508 longitudes).  By default
- * phases are fused with filtered rows while active members x num_lat x
- * ceil(num_lon/254) row tiles stay within 24576 (num_lon <= 1024) or 6144
- * (num_lon <= 2048), in form 2 while its workgroups fit two per compute
- * unit; gd_autotune may choose any form.  GD_FUSED=0, 1 or 2 in the
- * environment when the handle is created forces a form (2 falls back to 1
- * where rows are not of two segments).  Every form gives the same bits.  No
- * counterpart in the reference. */
+ * launch (tendency, filter rows and epilogue), 2 one fused launch in
+ * workgroups of two row segments (a filtered row's segments evaluated two at
+ * a time; rows of 255 to 762 longitudes).  By default phases are fused with
+ * filtered rows while active members x num_lat x ceil(num_lon/254) row
+ * tiles stay within 24576 (num_lon <= 1024) or 6144 (num_lon <= 2048), in
+ * form 2 for rows of 255 to 508 longitudes while its workgroups (filtered
+ * rows plus far rows) fit two per compute unit; gd_autotune may choose any
+ * form.  GD_FUSED=0, 1 or 2 in the environment when the handle is created
+ * forces a form (2 falls back to 1 outside 255 to 762 longitudes).  Every
+ * form gives the same bits.  No counterpart in the reference. */
 int gd_fused_phases(const gd_handle *h);
 
 /* Tile autotuner (no counterpart in the reference).  For the current

@@ -1,11 +1,13 @@
-"""The wide phase launch (k_wphase: one workgroup per row, its two segment
-tiles side by side; a filter row's segments evaluated at once and filtered
-with one longitude per thread) against the fused launch k_phase (GD_FUSED=2
-and GD_FUSED=1 when the handle is created).  Each tile does k_phase's
-arithmetic and the filter's sums keep filter_row's order, so the two must
-agree bit for bit: states, per-step mass, energy and beta, for rows of 255
-to 508 longitudes, even and odd num_lon, every split scheme, both
-precisions and an ensemble with a subset of active members.
+"""The wide phase launch (k_wphase: workgroups of two tiles; a filter row's
+segments evaluated two at a time and filtered with one or two longitudes
+per thread; row workgroups of two consecutive row tiles, of one row or two,
+with a spare tile past the last) against the fused launch k_phase
+(GD_FUSED=2 and GD_FUSED=1 when the handle is created).  Each tile does
+k_phase's arithmetic and the filter's sums keep filter_row's order, so the
+two must agree bit for bit: states, per-step mass, energy and beta, for rows
+of two and three segments (255 to 762 longitudes), even and odd num_lon,
+every split scheme, both precisions and an ensemble with a subset of active
+members.
 """
 import os
 from contextlib import contextmanager
@@ -58,6 +60,11 @@ def assert_same(a, b):
 
 CASES = [
     # (N, M, dt, steps, options)
+    (720, 361, 120.0, 3, dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=2.5e4)),
+    # three segments: workgroups pairing tiles of two rows, pole-row tiles
+    # paired with others (a spare tile: test_wide_phase_spare_tile)
+    (600, 91, 150.0, 3, dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_coef=4e4)),
+    (762, 37, 120.0, 2, dict(split_scheme="none", use_diffusion=True, diffusion_coef=2.5e4)),
     (360, 181, 240.0, 3, dict(split_scheme="csp2", subcycles=6, use_diffusion=True, diffusion_coef=1e5)),
     (360, 181, 240.0, 2, dict(split_scheme="isp", subcycles=3, use_diffusion=True, diffusion_coef=1e5)),
     (360, 91, 300.0, 3, dict(split_scheme="none", uv_adv_scheme="upwind")),
@@ -91,10 +98,35 @@ def test_wide_phase_ensemble_active_subset(gd):
                 run(gd, 1, N, M, 300.0, 3, members=3, active=2, **kw)[1])
 
 
-def test_wide_phase_only_for_two_segments(gd):
+def test_wide_phase_only_for_two_or_three_segments(gd):
     """GD_FUSED=2 falls back to k_phase where a row is one segment or more
-    than two."""
-    for N, M in ((180, 91), (720, 37), (1024, 33)):
+    than three."""
+    for N, M in ((180, 91), (763, 37), (1024, 33)):
         with form_env(2):
             with gd.Dycore(N, M, 60.0, zonal_tend_filter_cutoff_wavenumber=gd.cutoff_from_lat0(N, M, 60.0)) as d:
                 assert d.fused_phases() == 1
+
+
+def far_rows(d, M):
+    """Rows with a component outside the filter, and the pole rows: the rows
+    of the fused launches' row tiles."""
+    ff, hf, _, _ = d.filter_table()
+    return sum(1 for j in range(M) if j in (0, M - 1) or not ff[j] or (j <= M - 2 and not hf[j]))
+
+
+def test_wide_phase_spare_tile(gd):
+    """Rows of three segments and an odd number of far rows: an odd tile
+    count, so the last row workgroup has a spare tile."""
+    N = 600
+    kw = dict(split_scheme="csp2", subcycles=3, use_diffusion=True, diffusion_coef=4e4, precision="mixed")
+    for M in range(61, 101, 2):
+        cut = gd.cutoff_from_lat0(N, M, 60.0)
+        with gd.Dycore(N, M, 150.0, zonal_tend_filter_cutoff_wavenumber=cut, **kw) as d:
+            nf = far_rows(d, M)
+        if nf % 2 == 1:
+            break
+    else:
+        pytest.fail("no grid with an odd far-row count")
+    print(f"wide spare tile N={N} M={M} far rows {nf}", flush=True)
+    kw["zonal_tend_filter_cutoff_wavenumber"] = cut
+    assert_same(run(gd, 2, N, M, 150.0, 3, **kw)[1], run(gd, 1, N, M, 150.0, 3, **kw)[1])
