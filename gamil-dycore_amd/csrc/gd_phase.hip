@@ -168,8 +168,10 @@ k_phase(const Args a) {
 // 480x241 step 9-10% faster, two runs of 360x181 3-5%.  At 720x361 (three
 // segments) a single run steps at 0.83-0.85 of k_phase: its 633 workgroups
 // take a second round at two per CU, where k_phase's 1,087 tiles fit five
-// per CU.  Workgroups of three tiles (one per segment) were no better: one
-// twelve-wave workgroup per CU at 96 VGPRs, and two per CU (80 VGPRs) spill.
+// per CU; capped at 80 VGPRs (three per CU, one round, with spills) it
+// stepped at 0.82.  Workgroups of three tiles (one per segment) were no
+// better: one twelve-wave workgroup per CU at 96 VGPRs, and two per CU
+// (80 VGPRs) spill.
 //
 // LDS: kWTiles stencil regions of wphase_region bytes (region 0 later holds
 // FFT buffer b1, region 1 the filter's products), then the row buffer b0
