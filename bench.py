@@ -143,7 +143,8 @@ def pmc_probe(gd, ic, nsteps):
     print(f"pmc probe: {nsteps} steps done", flush=True)
 
 
-# ---- the other workloads (run with --workload; not the default line) -------
+# ---- the other workloads: config.other_workloads of the default line, or one
+# ---- line each with --workload
 
 def _cfg(gd, N, M, dt, coef, prec, members=1):
     cut = gd.cutoff_from_lat0(N, M, FILTER_LAT0)
@@ -210,6 +211,23 @@ def ensemble_workload(gd, steps, warmup, tune=True):
         d.close()
         out[prec] = mem * steps * dt / 86400.0 / el
     return out, _tend_roofline(N, M, mem, *kern["mixed"]), tuned
+
+
+def w360_workload(gd, steps, warmup):
+    """The 1-degree single run (360x181, dt 240 s, diffusion 1e5; the
+    reference's own test resolution): model-days/s in mixed precision and
+    FP64, the phase launch form, and the roofline of that phase launch."""
+    N, M, dt = 360, 181, 240.0
+    ic = gd.ic_balanced(N, M, seed=20240, nmax=20, slope=-3.0, rms_height=150.0, mean_depth=8000.0)
+    out, kern = {}, {}
+    for prec in ("mixed", "fp64"):
+        d = _cfg(gd, N, M, dt, 1.0e5, prec)
+        d.set_state(*ic)
+        el = _time_steps(d, steps, warmup)
+        kern[prec] = _kernel_us(d, 200)
+        d.close()
+        out[prec] = steps * dt / 86400.0 / el
+    return out, _tend_roofline(N, M, 1, *kern["mixed"])
 
 
 def w1440_workload(gd, steps, warmup):
@@ -582,6 +600,11 @@ def main():
             "autotuned": etuned, "roofline": {k: eroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
         if cpu is not None:
             extra["ensemble_64x360x181"]["over_cpu_processes"] = round(eo["mixed"] / cpu["ensemble"]["value"], 1)
+        so, sroof = w360_workload(gd, 360, 30)
+        extra["360x181"] = {
+            "model_days_per_s": round(so["mixed"], 3), "fp64_model_days_per_s": round(so["fp64"], 3),
+            "mixed_over_fp64": round(so["mixed"] / so["fp64"], 3), "steps": 360,
+            "roofline": {k: sroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
         wo, wroof = w1440_workload(gd, args.w1440_steps, 60)
         extra["1440x721"] = {
             "model_days_per_s": round(wo["mixed"], 4), "fp64_model_days_per_s": round(wo["fp64"], 4),
