@@ -94,6 +94,7 @@ EXPORTS = (
     "gd_restart_write", "gd_restart_read", "gd_io_last_error", "gd_ic_steady_geostrophic",
     "gd_ic_jet_zonal_flow", "gd_ic_shallow_water_waves", "gd_serve", "gd_serve_last_error",
     "gd_set_active_members", "gd_copy_member", "gd_fused_phases", "gd_filter_rows_noalloc",
+    "gd_check_antisymmetry",
     "gd_autotune", "gd_server_create", "gd_server_submit", "gd_server_poll", "gd_server_destroy",
 )
 
@@ -152,6 +153,7 @@ def lib():
     L.gd_set_active_members.argtypes = [vp, C.c_int32]
     L.gd_copy_member.argtypes = [vp, C.c_int32, C.c_int32]
     L.gd_fused_phases.argtypes = [vp]
+    L.gd_check_antisymmetry.argtypes = [vp, C.c_int32, dp, dp, dp]
     L.gd_serve.argtypes = [C.POINTER(ServeConfig), C.POINTER(ServeRequest), C.c_int32,
                            C.POINTER(ServeResult), dp]
     L.gd_serve_last_error.restype = C.c_char_p
@@ -492,6 +494,17 @@ class Dycore:
         du = np.zeros((self.M, self.N)); dv = np.zeros((self.M - 1, self.N)); dgd = np.zeros((self.M, self.N))
         _check(lib().gd_space_operators(self.h, member, PASS[pass_], _dp(du), _dp(dv), _dp(dgd), self.N), self.h)
         return du, dv, dgd
+
+    def check_antisymmetry(self, member=0):
+        """check_antisymmetry (dycore_mod.F90:794-851) of the member's current
+        state on the device.  Returns (terms, magnitudes, sums): the ten inner
+        products u_adv_lon, u_adv_lat, fv, u_pgf, v_adv_lon, v_adv_lat, fu,
+        v_pgf, mass_div_lon, mass_div_lat; the same sums of magnitudes; and
+        the four sums the reference prints (advection, u_pgf + mass_div_lon,
+        fv - fu, v_pgf + mass_div_lat)."""
+        terms, mags, sums = np.zeros(10), np.zeros(10), np.zeros(4)
+        _check(lib().gd_check_antisymmetry(self.h, member, _dp(terms), _dp(mags), _dp(sums)), self.h)
+        return terms, mags, sums
 
     def predict_correct(self, dt, pass_="all", qcon_modified=True):
         beta = np.zeros(self.members)

@@ -972,6 +972,31 @@ int gd_space_operators(gd_handle *h, int32_t member, int32_t pass, double *du, d
   return GD_OK;
 }
 
+int gd_check_antisymmetry(gd_handle *h, int32_t member, double *terms, double *magnitudes, double *sums) {
+  int rc = check_member(h, member);
+  if (rc) return rc;
+  hipSetDevice(h->device);
+  Args a = base_args(h);
+  a.m0 = member;
+  a.s_in = SX;
+  GD_HIP(h, launch_terms(a, h->stream));
+  double out[20];
+  GD_HIP(h, hipMemcpyAsync(out, hmember(h, member).tb(CDV), sizeof(out), hipMemcpyDeviceToHost, h->stream));
+  GD_HIP(h, hipStreamSynchronize(h->stream));
+  for (int k = 0; k < 10; ++k) {
+    if (terms) terms[k] = out[k];
+    if (magnitudes) magnitudes[k] = out[10 + k];
+  }
+  if (sums) {
+    // the four sums check_antisymmetry prints (dycore_mod.F90:846-850)
+    sums[0] = out[0] + out[4] + out[1] + out[5];
+    sums[1] = out[3] + out[8];
+    sums[2] = out[2] - out[6];
+    sums[3] = out[7] + out[9];
+  }
+  return GD_OK;
+}
+
 int gd_predict_correct(gd_handle *h, double dt, int32_t pass, int32_t qcon, double *beta) {
   if (!h || pass < 0 || pass > 2) return GD_ERR_INVALID_ARG;
   hipSetDevice(h->device);

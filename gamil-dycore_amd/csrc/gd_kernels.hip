@@ -417,6 +417,120 @@ __global__ void __launch_bounds__(kBlock) k_divvor(const Args a) {
 }
 
 // ---------------------------------------------------------------------------
+// check_antisymmetry (dycore_mod.F90:794-851): the ten inner products of the
+// operator terms of state X with the IAP state they act on
+// ---------------------------------------------------------------------------
+
+// One workgroup per full row j of member a.m0; thread tid takes longitudes
+// tid, tid + kBlock, ...  Each term is the reference operator's expression
+// (dycore_mod.F90:367-598, as the oracle restates it) in FP64 from the
+// state's u, v, U, V, gd and ghs, and enters as term * weight * cos
+// (u terms: iap u and full cos; v terms: iap v and half cos; mass terms:
+// gd + ghs and full cos).  Row j's ten signed sums and ten sums of
+// magnitudes go to scratch row j (Tb du field): [j][20].
+__global__ void __launch_bounds__(kBlock) k_terms(const Args a) {
+  const int j = blockIdx.x;
+  const int N = a.N, M = a.M, P = a.pitch;
+  const Member mb = member_at(a, a.m0);
+  __shared__ double red[4 * 20];
+  const double *gd = mb.st(SX, FGD), *U = mb.st(SX, FU), *V = mb.st(SX, FV);
+  const double *u = mb.st(SX, Fu), *v = mb.st(SX, Fv), *ghs = mb.ghs();
+  const RowCoef cj = a.coef[j];
+  const bool upwind = a.upwind != 0;
+  const double bl = a.beta_lon, bt = a.beta_lat;
+  // pole rows: the meridional mass divergence is the ring sum at every longitude
+  const bool pole = j == 0 || j == M - 1;
+  const double pflux = pole ? pole_flux(a, mb, j, red) : 0.0;
+  auto G = [&](int jj, int ii) { return sqrt(gd[(size_t)jj * P + ii]); };
+  auto at = [&](const double *f, int jj, int ii) { return f[(size_t)jj * P + ii]; };
+  double q[20];
+#pragma unroll
+  for (int k = 0; k < 20; ++k) q[k] = 0.0;
+  auto add = [&](int k, double t, double w, double c) {
+    const double x = t * w * c;
+    q[k] += x;
+    q[10 + k] += fabs(x);
+  };
+  for (int i = threadIdx.x; i < N; i += kBlock) {
+    const int im = i == 0 ? N - 1 : i - 1, ip = i == N - 1 ? 0 : i + 1;
+    // ---- u terms, full row j (interior rows; zero at the poles) ----
+    if (j >= 1 && j <= M - 2) {
+      const double U0 = at(U, j, i), Up = at(U, j, ip), Um = at(U, j, im);
+      const double u1 = at(u, j, i) + at(u, j, im), u2 = at(u, j, i) + at(u, j, ip);
+      const double ual = upwind ? 0.25 / cj.fdlon * (u2 * (U0 + Up) - bl * fabs(u2) * (Up - U0) - u1 * (U0 + Um) +
+                                                     bl * fabs(u1) * (U0 - Um) - (u2 - u1) * U0)
+                                : 0.25 / cj.fdlon * (u2 * Up - u1 * Um);
+      const double hcm = a.coef[j - 1].hcos;
+      const double v1 = (at(v, j - 1, i) + at(v, j - 1, ip)) * hcm, v2 = (at(v, j, i) + at(v, j, ip)) * cj.hcos;
+      const double Ujp = at(U, j + 1, i), Ujm = at(U, j - 1, i);
+      const double uat = upwind ? 0.25 / cj.fdlat * (v2 * (U0 + Ujp) - bt * fabs(v2) * (Ujp - U0) - v1 * (U0 + Ujm) +
+                                                     bt * fabs(v1) * (U0 - Ujm) - (v2 - v1) * U0)
+                                : 0.25 / cj.fdlat * (v2 * Ujp - v1 * Ujm);
+      const double fv = 0.25 * (cj.f + cj.c * at(u, j, i)) *
+                        (cj.c1 * (at(V, j - 1, i) + at(V, j - 1, ip)) + cj.c2 * (at(V, j, i) + at(V, j, ip)));
+      const double upgf = 0.5 * (G(j, i) + G(j, ip)) / cj.fdlon *
+                          (at(gd, j, ip) + at(ghs, j, ip) - at(gd, j, i) - at(ghs, j, i));
+      add(0, ual, U0, cj.fcos);
+      add(1, uat, U0, cj.fcos);
+      add(2, fv, U0, cj.fcos);
+      add(3, upgf, U0, cj.fcos);
+    }
+    // ---- v terms, half row j ----
+    if (j <= M - 2) {
+      const RowCoef cn = a.coef[j + 1];
+      const double V0 = at(V, j, i), Vp = at(V, j, ip), Vm = at(V, j, im);
+      const double u1 = at(u, j, im) + at(u, j + 1, im), u2 = at(u, j, i) + at(u, j + 1, i);
+      const double val = upwind ? 0.25 / cj.hdlon * (u2 * (V0 + Vp) - bl * fabs(u2) * (Vp - V0) - u1 * (V0 + Vm) +
+                                                     bl * fabs(u1) * (V0 - Vm) - (u2 - u1) * V0)
+                                : 0.25 / cj.hdlon * (u2 * Vp - u1 * Vm);
+      // v and V beyond the half rows are zero, as are the cosines there
+      const double hcm = j >= 1 ? a.coef[j - 1].hcos : 0.0, hcp = j + 1 <= M - 2 ? cn.hcos : 0.0;
+      const double vm = j >= 1 ? at(v, j - 1, i) : 0.0, vp = j + 1 <= M - 2 ? at(v, j + 1, i) : 0.0;
+      const double Vjm = j >= 1 ? at(V, j - 1, i) : 0.0, Vjp = j + 1 <= M - 2 ? at(V, j + 1, i) : 0.0;
+      const double v1 = at(v, j, i) * cj.hcos + vm * hcm, v2 = at(v, j, i) * cj.hcos + vp * hcp;
+      const double vat = upwind ? 0.25 / cj.hdlat * (v2 * (V0 + Vjp) - bt * fabs(v2) * (Vjp - V0) - v1 * (V0 + Vjm) +
+                                                     bt * fabs(v1) * (V0 - Vjm) - (v2 - v1) * V0)
+                                : 0.25 / cj.hdlat * (v2 * Vjp - v1 * Vjm);
+      const double fu = 0.25 * ((cj.f + cj.c * at(u, j, i)) * at(U, j, i) + (cj.f + cj.c * at(u, j, im)) * at(U, j, im) +
+                                (cn.f + cn.c * at(u, j + 1, i)) * at(U, j + 1, i) +
+                                (cn.f + cn.c * at(u, j + 1, im)) * at(U, j + 1, im));
+      const double vpgf = 0.5 * (G(j, i) + G(j + 1, i)) / cj.hdlat * cj.hcos *
+                          (at(gd, j + 1, i) + at(ghs, j + 1, i) - at(gd, j, i) - at(ghs, j, i));
+      add(4, val, V0, cj.hcos);
+      add(5, vat, V0, cj.hcos);
+      add(6, fu, V0, cj.hcos);
+      add(7, vpgf, V0, cj.hcos);
+    }
+    // ---- mass divergence, full row j ----
+    {
+      double mdl = 0.0, mdp = pflux;
+      if (!pole) {
+        const double Gi = G(j, i);
+        mdl = ((Gi + G(j, ip)) * at(U, j, i) - (Gi + G(j, im)) * at(U, j, im)) * 0.5 / cj.fdlon;
+        mdp = ((Gi + G(j + 1, i)) * at(V, j, i) * cj.hcos - (Gi + G(j - 1, i)) * at(V, j - 1, i) * a.coef[j - 1].hcos) *
+              0.5 / cj.fdlat;
+      }
+      const double wg = at(gd, j, i) + at(ghs, j, i);
+      add(8, mdl, wg, cj.fcos);
+      add(9, mdp, wg, cj.fcos);
+    }
+  }
+  block_sum_n<20>(q, red);
+  if (threadIdx.x < 20) mb.tb(CDU)[(size_t)j * 20 + threadIdx.x] = q[threadIdx.x];
+}
+
+// The rows' sums added in row order, one thread per value: the twenty
+// totals to the first entries of the Tb dv field.
+__global__ void __launch_bounds__(64) k_terms_sum(const Args a) {
+  const Member mb = member_at(a, a.m0);
+  const int k = threadIdx.x;
+  if (k >= 20) return;
+  double s = 0.0;
+  for (int j = 0; j < a.M; ++j) s += mb.tb(CDU)[(size_t)j * 20 + k];
+  mb.tb(CDV)[k] = s;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
@@ -486,6 +600,14 @@ hipError_t launch_finish(const Args &a, int mode, hipStream_t s) {
 
 hipError_t launch_diag(const Args &a, int nmem, hipStream_t s) {
   hipLaunchKernelGGL(k_diag, dim3(nmem), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_terms(const Args &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_terms, dim3(a.M), dim3(kBlock), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_terms_sum, dim3(1), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 

@@ -34,6 +34,9 @@ hipError_t launch_diffuse(const Args &a, int prec, hipStream_t s);
 hipError_t launch_finish(const Args &a, int mode, hipStream_t s);
 hipError_t launch_diag(const Args &a, int nmem, hipStream_t s);
 hipError_t launch_divvor(const Args &a, hipStream_t s);
+// check_antisymmetry's ten inner products (and their magnitudes) of member
+// a.m0's state X, left in its Tb dv field (entries 0..19; Tb du is scratch)
+hipError_t launch_terms(const Args &a, hipStream_t s);
 hipError_t configure_kernels(int max_lds);
 // polar filter of the staged rows + rescale + phase epilogue (gd_filter.hip)
 hipError_t launch_filter(const Args &a, int prec, hipStream_t s);

@@ -132,6 +132,20 @@ int gd_get_diag_fields(gd_handle *h, int32_t member, double *div, double *vor,
 
 /* ---- operator seams (the reference's internal procedures) --------------- */
 
+/* check_antisymmetry(tend, state) (dycore_mod.F90:794-851) of the member's
+ * current state, evaluated on the device: the ten inner products of the
+ * operator terms with the state (u_adv_lon, u_adv_lat, fv, u_pgf with iap u
+ * and full cos; v_adv_lon, v_adv_lat, fu, v_pgf with iap v and half cos;
+ * mass_div_lon, mass_div_lat with gd + ghs and full cos; the operators of
+ * dycore_mod.F90:367-598 before the filter) into terms[10], the same sums of
+ * magnitudes into magnitudes[10], and the four sums the reference prints,
+ * which vanish to round-off for the center scheme, into sums[4]: advection,
+ * u_pgf + mass_div_lon, fv - fu, v_pgf + mass_div_lat.  Any pointer may be
+ * NULL.  Uses the member's corrector scratch, which holds nothing between
+ * steps; the state is not changed. */
+int gd_check_antisymmetry(gd_handle *h, int32_t member, double *terms,
+                          double *magnitudes, double *sums);
+
 /* space_operators(state(old), tend, pass) (dycore_mod.F90:184-365) including
  * the polar filter and its inner-product rescale; du (num_lat rows, pole rows
  * 0), dv (num_lat-1 rows), dgd (num_lat rows).  The state is not changed.
