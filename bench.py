@@ -117,7 +117,7 @@ def run_batched(gd, prec, ic, members, steps):
     return members * steps * WORKLOAD["time_step_size"] / 86400.0 / el, el
 
 
-def pmc_probe(gd, ic, nsteps):
+def pmc_probe(gd, ic, nsteps, grid=720):
     """Workload of the hardware-counter runs: a plain device-to-device copy of
     256 MiB through the HIP runtime (its FETCH_SIZE / WRITE_SIZE per dispatch
     calibrate the byte counters), then nsteps steps of the mixed workload.
@@ -136,11 +136,15 @@ def pmc_probe(gd, ic, nsteps):
     print(f"copy calibration: {nbytes} bytes read and written", flush=True)
     hip.hipFree(src)
     hip.hipFree(dst)
-    d, _ = make_dycore(gd, "mixed")
-    d.set_state(*ic)
+    if grid == 360:  # the 1-degree single run (rows of two segments)
+        d = _cfg(gd, 360, 181, 240.0, 1.0e5, "mixed")
+        d.set_state(*gd.ic_balanced(360, 181, seed=20240, nmax=20, slope=-3.0, rms_height=150.0, mean_depth=8000.0))
+    else:
+        d, _ = make_dycore(gd, "mixed")
+        d.set_state(*ic)
     d.step(nsteps)
     d.close()
-    print(f"pmc probe: {nsteps} steps done", flush=True)
+    print(f"pmc probe: {nsteps} steps of {grid} longitudes done", flush=True)
 
 
 # ---- the other workloads: config.other_workloads of the default line, or one
@@ -498,6 +502,8 @@ def main():
     ap.add_argument("--pmc-probe", type=int, default=0, metavar="K",
                     help="counter runs: one 256 MiB device copy (calibration of the byte "
                          "counters) then K steps of the mixed workload, no timing output")
+    ap.add_argument("--probe-grid", type=int, default=720, choices=[720, 360],
+                    help="grid of the --pmc-probe steps: the 720x361 workload or the 360x181 single run")
     ap.add_argument("--mm-probe", type=int, default=0, metavar="K",
                     help="counter runs: K steps of the 64-member ensemble in the two matrix-core "
                          "filter modes, no timing output")
@@ -524,7 +530,7 @@ def main():
     N, M, dt = WORKLOAD["num_lon"], WORKLOAD["num_lat"], WORKLOAD["time_step_size"]
     ic = gd.ic_balanced(N, M, **IC)
     if args.pmc_probe:
-        pmc_probe(gd, ic, args.pmc_probe)
+        pmc_probe(gd, ic, args.pmc_probe, args.probe_grid)
         return
     if args.mm_probe:
         # counter runs of the matrix-core filter: K steps of the 64-member
