@@ -135,6 +135,7 @@ struct Args {
   int in_full;                 // k_diffuse: read stored u, v of s_in
   int nslots;
   int upd_wpm, nmem_launch;    // k_update: workgroups per member, members
+  int upd_narrow;              // k_update: four points and partials per thread per round (else eight)
   int lap_src, lap_final;      // k_diffuse: operand (0 state s_in, 1 slot s_lap), last application
   int s_lap, s_lapout;         // k_diffuse: operand and result slots of intermediate applications
   int isp_mode, upd_isp;       // k_isp mode; k_update applies the isp corrector

@@ -46,8 +46,14 @@ __global__ void __launch_bounds__(kBlock) k_tend(const Args a) {
 
 // upd_wpm workgroups per member.  Each forms the member's beta from the
 // phase-C partials (same loads, same order in every workgroup, so the same
-// bits), then applies S_out = S0 + dt beta Tc to its band of rows.
-template <typename CT>
+// bits), then applies S_out = S0 + dt beta Tc to its band of rows.  U:
+// points per thread per batch of the update, PU: partials per thread per
+// load round.  A thread adds its partials k = tid, tid + kBlock, ... in that
+// order whatever PU is, so every (U, PU) gives the same bits; U = PU = 8
+// holds two 720-point rows in one batch (one round trip for a single run),
+// U = PU = 4 about half the registers (more workgroups resident for
+// batches of members).
+template <typename CT, int U, int PU>
 __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   const int wpm = a.upd_wpm;
   const int m = a.m0 + blockIdx.x / wpm, w = blockIdx.x - (blockIdx.x / wpm) * wpm;
@@ -66,7 +72,6 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   const double *tcu = mb.tc(CDU), *tcv = mb.tc(CDV);
   const double *rcu = mb.st(SP2, Fu), *rcv = mb.st(SP2, Fv);
   double *og = mb.st(a.s_out, FGD), *ou = mb.st(a.s_out, FU), *ov = mb.st(a.s_out, FV);
-  constexpr int U = 8;  // points per thread per batch (two 720-point rows in one)
   size_t off[U];
   bool on[U], hv[U];
   double g0[U], u0[U], v0[U], tg[U], tu[U], tv[U];
@@ -97,9 +102,9 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
   // (tend(new), tend(new)) (types_mod.F90:347-371) summed du, then dv, then dgd
   const int per_comp = M * nseg;
   double q[6] = {0, 0, 0, 0, 0, 0};
-  auto pload = [&](int k0, double(&x)[8][6]) {
+  auto pload = [&](int k0, double(&x)[PU][6]) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < PU; ++u) {
       const int k = k0 + u * kBlock + tid;
       const bool in = k < per_comp;
       const int kk = in ? k : 0;  // unconditional loads, as in load()
@@ -112,9 +117,9 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
       }
     }
   };
-  auto pacc = [&](const double(&x)[8][6]) {
+  auto pacc = [&](const double(&x)[PU][6]) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < PU; ++u)
 #pragma unroll
       for (int c = 0; c < 6; ++c) q[c] += x[u][c];
   };
@@ -122,13 +127,13 @@ __global__ void __launch_bounds__(kBlock) k_update(const Args a) {
     // the first partials, then the band's first batch, which does not depend
     // on beta: memory returns in issue order, so the reduction waits only
     // for the partials and overlaps the batch's loads
-    double x[8][6];
+    double x[PU][6];
     pload(0, x);
     load(0);
     pacc(x);
   }
-  for (int k0 = 8 * kBlock; k0 < per_comp; k0 += 8 * kBlock) {
-    double x[8][6];
+  for (int k0 = PU * kBlock; k0 < per_comp; k0 += PU * kBlock) {
+    double x[PU][6];
     pload(k0, x);
     pacc(x);
   }
@@ -569,10 +574,17 @@ hipError_t launch_isp(const Args &a, int prec, hipStream_t s) {
 
 hipError_t launch_update(const Args &a, int prec, hipStream_t s) {
   dim3 g(a.upd_wpm * a.nmem_launch), b(kBlock);
-  if (prec == 0)
-    hipLaunchKernelGGL(k_update<double>, g, b, 0, s, a);
-  else
-    hipLaunchKernelGGL(k_update<float>, g, b, 0, s, a);
+  if (a.upd_narrow) {
+    if (prec == 0)
+      hipLaunchKernelGGL((k_update<double, 4, 4>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_update<float, 4, 4>), g, b, 0, s, a);
+  } else {
+    if (prec == 0)
+      hipLaunchKernelGGL((k_update<double, 8, 8>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_update<float, 8, 8>), g, b, 0, s, a);
+  }
   return hipGetLastError();
 }
 
