@@ -395,8 +395,9 @@ int64_t gd_device_bytes(const gd_handle *h);
  * state X is not modified).  kernel: GD_TIME_TEND (tendency kernel, fast
  * pass, second predictor), GD_TIME_FILTER (the filter of that phase),
  * GD_TIME_UPDATE (beta + corrector), GD_TIME_PHASE (that phase as one fused
- * launch: tendency, filter rows and epilogue; GD_ERR_NOT_SUPPORTED when the
- * handle steps with separate tendency and filter launches), GD_TIME_FILTER_MM
+ * launch, in the form gd_fused_phases reports: tendency, filter rows and
+ * epilogue; GD_ERR_NOT_SUPPORTED when the handle steps with separate
+ * tendency and filter launches), GD_TIME_FILTER_MM
  * (the matrix-core projection of the momentum rows of every member; the
  * mixed_mfma and mixed_bf16 modes only).  No counterpart in the reference: a
  * measurement hook. */
