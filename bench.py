@@ -50,12 +50,12 @@ def tend_launch_bytes(N, M, prec):
     return state + 8 * full + state + state + tend
 
 
-def traffic_from_profile(kernel):
+def traffic_from_profile(kernel, grid="720x361"):
     """HBM bytes per launch of the same kernel launch measured with
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a 256 MiB copy), as
-    committed in profiles/<kernel>_720x361_traffic.txt; None when the
+    committed in profiles/<kernel>_<grid>_traffic.txt; None when the
     summary is absent."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_720x361_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{grid}_traffic.txt")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -182,8 +182,9 @@ def _tend_roofline(N, M, members, us, fused):
     kname = KERNEL_OF_FORM[fused]
     return {"bound": "hbm", "kernel": f"{kname}<float, fast pass> (second predictor)",
             "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4), "traffic": None, "bytes_per_launch": int(nbytes),
-            "avg_launch_us": round(us, 3)}
+            "frac": round(achieved / 8000.0, 4),
+            "traffic": traffic_from_profile(kname, f"{N}x{M}") if members == 1 else None,
+            "bytes_per_launch": int(nbytes), "avg_launch_us": round(us, 3)}
 
 
 def _ensemble_ics(gd, N, M, members):
@@ -610,7 +611,7 @@ def main():
         extra["360x181"] = {
             "model_days_per_s": round(so["mixed"], 3), "fp64_model_days_per_s": round(so["fp64"], 3),
             "mixed_over_fp64": round(so["mixed"] / so["fp64"], 3), "steps": 360,
-            "roofline": {k: sroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
+            "roofline": {k: sroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us", "traffic")}}
         wo, wroof = w1440_workload(gd, args.w1440_steps, 60)
         extra["1440x721"] = {
             "model_days_per_s": round(wo["mixed"], 4), "fp64_model_days_per_s": round(wo["fp64"], 4),
