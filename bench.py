@@ -117,7 +117,7 @@ def run_batched(gd, prec, ic, members, steps):
     return members * steps * WORKLOAD["time_step_size"] / 86400.0 / el, el
 
 
-def pmc_probe(gd, ic, nsteps, grid=720):
+def pmc_probe(gd, ic, nsteps, grid="720"):
     """Workload of the hardware-counter runs: a plain device-to-device copy of
     256 MiB through the HIP runtime (its FETCH_SIZE / WRITE_SIZE per dispatch
     calibrate the byte counters), then nsteps steps of the mixed workload.
@@ -136,15 +136,22 @@ def pmc_probe(gd, ic, nsteps, grid=720):
     print(f"copy calibration: {nbytes} bytes read and written", flush=True)
     hip.hipFree(src)
     hip.hipFree(dst)
-    if grid == 360:  # the 1-degree single run (rows of two segments)
+    if grid == "360":  # the 1-degree single run (rows of two segments)
         d = _cfg(gd, 360, 181, 240.0, 1.0e5, "mixed")
         d.set_state(*gd.ic_balanced(360, 181, seed=20240, nmax=20, slope=-3.0, rms_height=150.0, mean_depth=8000.0))
+    elif grid == "1440":  # the 0.25-degree single run
+        d = _cfg(gd, 1440, 721, 60.0, 6.25e3, "mixed")
+        d.set_state(*gd.ic_balanced(1440, 721, seed=20242, nmax=80))
+    elif grid == "ens64":  # the 64-member 360x181 ensemble, default launch shapes
+        d = _cfg(gd, ENS["N"], ENS["M"], ENS["dt"], ENS["coef"], "mixed", members=ENS["members"])
+        for k, icm in enumerate(_ensemble_ics(gd, ENS["N"], ENS["M"], ENS["members"])):
+            d.set_state(*icm, member=k)
     else:
         d, _ = make_dycore(gd, "mixed")
         d.set_state(*ic)
     d.step(nsteps)
     d.close()
-    print(f"pmc probe: {nsteps} steps of {grid} longitudes done", flush=True)
+    print(f"pmc probe: {nsteps} steps of workload {grid} done", flush=True)
 
 
 # ---- the other workloads: config.other_workloads of the default line, or one
@@ -183,7 +190,7 @@ def _tend_roofline(N, M, members, us, fused):
     return {"bound": "hbm", "kernel": f"{kname}<float, fast pass> (second predictor)",
             "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4),
-            "traffic": traffic_from_profile(kname, f"{N}x{M}") if members == 1 else None,
+            "traffic": traffic_from_profile(kname, f"{N}x{M}" if members == 1 else f"ens{members}x{N}x{M}"),
             "bytes_per_launch": int(nbytes), "avg_launch_us": round(us, 3)}
 
 
@@ -503,8 +510,9 @@ def main():
     ap.add_argument("--pmc-probe", type=int, default=0, metavar="K",
                     help="counter runs: one 256 MiB device copy (calibration of the byte "
                          "counters) then K steps of the mixed workload, no timing output")
-    ap.add_argument("--probe-grid", type=int, default=720, choices=[720, 360],
-                    help="grid of the --pmc-probe steps: the 720x361 workload or the 360x181 single run")
+    ap.add_argument("--probe-grid", default="720", choices=["720", "360", "1440", "ens64"],
+                    help="workload of the --pmc-probe steps: 720x361, the 360x181 or 1440x721 single run, "
+                         "or the 64-member 360x181 ensemble")
     ap.add_argument("--mm-probe", type=int, default=0, metavar="K",
                     help="counter runs: K steps of the 64-member ensemble in the two matrix-core "
                          "filter modes, no timing output")
@@ -604,7 +612,7 @@ def main():
         extra["ensemble_64x360x181"] = {
             "member_days_per_s": round(eo["mixed"], 3), "fp64_member_days_per_s": round(eo["fp64"], 3),
             "mixed_over_fp64": round(eo["mixed"] / eo["fp64"], 3), "steps": args.ens_steps,
-            "autotuned": etuned, "roofline": {k: eroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
+            "autotuned": etuned, "roofline": {k: eroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us", "traffic")}}
         if cpu is not None:
             extra["ensemble_64x360x181"]["over_cpu_processes"] = round(eo["mixed"] / cpu["ensemble"]["value"], 1)
         so, sroof = w360_workload(gd, 360, 30)
@@ -616,7 +624,7 @@ def main():
         extra["1440x721"] = {
             "model_days_per_s": round(wo["mixed"], 4), "fp64_model_days_per_s": round(wo["fp64"], 4),
             "mixed_over_fp64": round(wo["mixed"] / wo["fp64"], 3), "steps": args.w1440_steps,
-            "roofline": {k: wroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us")}}
+            "roofline": {k: wroof[k] for k in ("kernel", "achieved", "frac", "avg_launch_us", "traffic")}}
         extra["filter_implementations"] = filter_crossover(gd, 120, 240)
         sv = serving_workload(gd, argparse.Namespace(serve_requests=args.serve_short), {})
         extra["serving"] = {"model_days_per_s": sv["value"], "p50_s": sv["latency_s"]["p50"],
