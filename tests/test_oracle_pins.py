@@ -22,9 +22,12 @@ slip in the restatement fails it:
   * the ghs term of the energy (diag_mod.F90:115-119): a lake at rest over
     mountains;
   * isp_splitting (dycore_mod.F90:689-752): the final update's identity
-    <D, D> = 2 <D, F> (D = F_new - F) under the qcon beta, and its combined
+    <D, D> = 2 <D, F> (D = F_new - F) under the qcon beta, its combined
     tendency R equal to twice the midpoint tendency L(F + dt/2 L(F)) to
-    O(dt^2).
+    O(dt^2), and, for a small perturbation of the resting state, the closed
+    form of the fast subcycles with the linearised fast operator.  The sign
+    of the first slow correction stays unpinned: it acts only at third order
+    around rest and at O(dt^2) in R.
 """
 import numpy as np
 import pytest
@@ -361,3 +364,89 @@ def test_isp_tendency_is_twice_the_midpoint_tendency(oracle_mod):
         errs.append(np.sqrt(_ip(N, M, diff, diff) / _ip(N, M, L, L)))
     assert errs[0] < 1e-2
     assert 3.0 < errs[0] / errs[1] < 5.5
+
+
+def _fast_jacobian(oracle_mod, N, M, gd0, eps):
+    """The fast operator's Jacobian at rest (U = V = 0, gd = gd0 everywhere,
+    no topography), column by column: central differences of
+    space_operators(FAST_PASS) at rest +- eps e_k.  Each perturbed state is
+    formed by update_state from a tendency holding eps e_k, so its u, v are
+    derived as in the model; the columns carry an O(eps^2) error."""
+    T = oracle_mod
+    o = T.Oracle(N, M, 600.0, use_zonal_tend_filter=False)
+    o.set_initial_state(np.zeros((M, N)), np.zeros((M - 1, N)), np.full((M, N), gd0), np.zeros((M, N)))
+    o.start()
+    tf = (T.T_DU, T.T_DV, T.T_DGD)
+    rows = (M, M - 1, M)
+    n = N * sum(rows)
+
+    def fast(x):
+        k = 0
+        for f, r in zip(tf, rows):
+            part = np.ascontiguousarray(x[k:k + r * N].reshape(r, N))
+            T.lib().og_set_field(o.h, 1, 0, f, part)
+            k += r * N
+        o.update_state(1.0, 0, o.old, o.new)
+        o.space_operators(o.new, 1, T.FAST_PASS)
+        return np.concatenate([o.tend(1, f).ravel() for f in tf])
+
+    J = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = eps
+        J[:, k] = (fast(e) - fast(-e)) / (2.0 * eps)
+    return J
+
+
+def test_isp_fast_subcycles_match_their_linear_closed_form(oracle_mod):
+    """A small perturbation x of the resting state: the slow operator
+    (advection) is quadratic in it and drops out, the fast one is linear,
+    L1 = J x.  Without qcon_modified one isp step then gives, to first
+    order in x (dycore_mod.F90:689-752; h = dt/m):
+      P2_k = (I + h/2 J + h^2/4 J^2) P_k,   P_k+1 = P_k + h J P2_k,
+      F_new - F = 2 R,   R = 2/m sum_k J P2_k,   P_0 = x.
+    The fast stages' fractions (h/2, h/2, h) each change the result at the
+    scale (h |J|)^2 of the residual allowed here."""
+    T = oracle_mod
+    N, M, m, dt = 12, 7, 2, 3000.0
+    gd0 = G * 8000.0
+    J = _fast_jacobian(oracle_mod, N, M, gd0, 1e-3)
+    rng = np.random.default_rng(4)
+    lat = np.linspace(-np.pi / 2, np.pi / 2, M)
+    lon = np.arange(N) * 2 * np.pi / N
+    hl = (np.linspace(-np.pi / 2, np.pi / 2, M)[:-1] + np.pi / (2 * (M - 1)))
+    amp = 1e-3
+    u = amp * np.cos(lat)[:, None] * (np.sin(lon + rng.uniform(0, 6))[None, :] + 0.5)
+    u[0] = u[-1] = 0.0
+    v = amp * np.cos(hl)[:, None] * np.cos(2 * lon + rng.uniform(0, 6))[None, :]
+    gd = gd0 + G * amp * np.cos(lat)[:, None] ** 2 * np.cos(lon + rng.uniform(0, 6))[None, :]
+    o = T.Oracle(N, M, dt, split_scheme="isp", subcycles=m, qcon_modified=False, use_zonal_tend_filter=False)
+    o.set_initial_state(u, v, gd, np.zeros((M, N)))
+    o.start()
+    F = [o.state(o.old, f).copy() for f in (T.IAP_U, T.IAP_V, T.GD)]
+    o.time_integrate()
+    Fn = [o.state(o.new, f) for f in (T.IAP_U, T.IAP_V, T.GD)]
+    x = np.concatenate([F[0].ravel(), F[1].ravel(), (F[2] - gd0).ravel()])
+    step = np.concatenate([(b - a).ravel() for a, b in zip(F, Fn)])
+    h = dt / m
+    I = np.eye(len(x))
+    A2 = I + 0.5 * h * J + 0.25 * h * h * (J @ J)
+    P = x.copy()
+    R = np.zeros_like(x)
+    for _ in range(m):
+        P2 = A2 @ P
+        R += J @ P2
+        P = P + h * (J @ P2)
+    pred = 2.0 * (2.0 / m) * R
+    # the same closed form with the first stage at h instead of h/2
+    P, Rs = x.copy(), np.zeros_like(x)
+    A2s = I + 0.5 * h * J + 0.5 * h * h * (J @ J)
+    for _ in range(m):
+        P2 = A2s @ P
+        Rs += J @ P2
+        P = P + h * (J @ P2)
+    slipped = 2.0 * (2.0 / m) * Rs
+    err = np.linalg.norm(step - pred) / np.linalg.norm(pred)
+    gap = np.linalg.norm(slipped - pred) / np.linalg.norm(pred)
+    assert gap > 1e-3            # the closed form resolves the stage fraction
+    assert err < 1e-2 * gap, (err, gap)
