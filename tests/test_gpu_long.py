@@ -76,7 +76,13 @@ PRECS = ["fp64", "mixed", "mixed_mfma", "mixed_bf16"]
 @pytest.mark.parametrize("name", CASES)
 def test_config_vs_oracle(gd, name, prec):
     if prec.startswith("mixed_") and name == "360x181_1day":
-        pytest.skip("the matrix-core filter modes run on the 1440x721, 720x361 and ensemble cases")
+        pytest.skip("the matrix-core filter modes run on the 720x361 and ensemble cases")
+    if prec.startswith("mixed_") and name == "1440x721_2day":
+        # one run has one row per projector, so 15 of every 16 matrix-core
+        # rows are padding: about 69 ms (FP32) / 17 ms (BF16) per step, 3.3
+        # minutes for the FP32 run; the 1440x721 filter table in these modes
+        # is checked by test_gpu_mfma_filter.py::test_mfma_projector_vs_oracle
+        pytest.skip("the matrix-core filter modes run on the 720x361 and ensemble cases")
     p, f = _fixture(name)
     s = p["stride"]
     (u, v, h), m0, mass, energy = _device_run(gd, p, prec)
