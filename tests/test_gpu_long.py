@@ -83,6 +83,12 @@ def test_config_vs_oracle(gd, name, prec):
         # minutes for the FP32 run; the 1440x721 filter table in these modes
         # is checked by test_gpu_mfma_filter.py::test_mfma_projector_vs_oracle
         pytest.skip("the matrix-core filter modes run on the 720x361 and ensemble cases")
+    if prec == "mixed_mfma" and name == "720x361_10day":
+        # 7200 steps at about 17 ms (one run: 15 of 16 matrix-core rows are
+        # padding) take two minutes; the mode runs at full size on the
+        # 64-member ensemble cases, and the BF16 mode on this one (measured
+        # here once: height l2 1.6e-7, mass drift 0)
+        pytest.skip("FP32 matrix-core filter: full-size run on the ensemble cases")
     p, f = _fixture(name)
     s = p["stride"]
     (u, v, h), m0, mass, energy = _device_run(gd, p, prec)
