@@ -166,12 +166,13 @@ k_phase(const Args a) {
 // Rows of two segments (255 to 508 longitudes) map one row to a workgroup.
 // Measured on MI355X against k_phase: single runs of 288x145, 360x181 and
 // 480x241 step 9-10% faster, two runs of 360x181 3-5%.  At 720x361 (three
-// segments) a single run steps at 0.83-0.85 of k_phase: its 633 workgroups
-// take a second round at two per CU, where k_phase's 1,087 tiles fit five
-// per CU; capped at 80 VGPRs (three per CU, one round, with spills) it
-// stepped at 0.82.  Workgroups of three tiles (one per segment) were no
-// better: one twelve-wave workgroup per CU at 96 VGPRs, and two per CU
-// (80 VGPRs) spill.
+// segments) a single run steps at 0.83-0.85 of k_phase: the filter
+// workgroups' chain is no shorter there (the first round of stencil loads
+// dominates the tendency stage, and k_phase already overlaps the later
+// segments' loads with evaluation), and the 633 workgroups take a second
+// round at two per CU, where k_phase's 1,087 tiles fit five per CU; capped
+// at 80 VGPRs (three per CU, with spills) it stepped at 0.82.  Workgroups of
+// three tiles (one per segment) were no better.
 //
 // LDS: kWTiles stencil regions of wphase_region bytes (region 0 later holds
 // FFT buffer b1, region 1 the filter's products), then the row buffer b0
