@@ -63,12 +63,6 @@ __device__ __forceinline__ void block_sum_n(double (&v)[K], double *red) {
   }
 }
 
-// Sum over the 16 lanes of a lane group (lane % 16 varies), deterministic.
-__device__ __forceinline__ double group16_sum(double v) {
-  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 __device__ __forceinline__ double fld_base(const Member &mb, int slot, int comp, size_t o) {
   return mb.st(slot, comp == CDU ? FU : (comp == CDV ? FV : FGD))[o];
 }
@@ -84,28 +78,8 @@ __device__ __forceinline__ void tend_store(const Member &mb, bool tb, int comp, 
   (tb ? mb.tb(comp) : mb.tc(comp))[o] = v;
 }
 
-// Epilogue of one tendency value of a predict/correct phase (not diffusion).
-// Returns the (Tb*T, T*T) products of phase C for the beta partials.
-template <typename CT>
-__device__ __forceinline__ void phase_epilogue(const Args &a, const Member &mb, int comp, int row,
-                                               int i, double T, double w, double &p1, double &p2) {
-  const size_t o = (size_t)row * a.pitch + i;
-  const int f = comp == CDU ? FU : (comp == CDV ? FV : FGD);
-  if (a.phase == PH_A || a.phase == PH_B) {
-    mb.st(a.s_out, f)[o] = mb.st(a.s0, f)[o] + a.hdt * T;
-    if (a.phase == PH_B) tend_store(mb, true, comp, o, T);
-  } else {
-    tend_store(mb, false, comp, o, T);
-    if (a.phase == PH_C) {
-      double tbv = tend_load(mb, true, comp, o);
-      p1 += tbv * T * w;
-      p2 += T * T * w;
-    }
-  }
-}
-
-
-// The epilogue split in two so a kernel can issue the load early: the value
+// The epilogue of one tendency value of a predict/correct phase (not
+// diffusion), split in two so a kernel can issue the load early: the value
 // phase_store needs (state s0 for the predictors, Tb for phase C) ...
 template <typename CT>
 __device__ __forceinline__ double phase_pre(const Args &a, const Member &mb, int comp, size_t o) {
@@ -114,7 +88,9 @@ __device__ __forceinline__ double phase_pre(const Args &a, const Member &mb, int
   return 0.0;
 }
 
-// ... and the stores of phase_epilogue given that value.
+// ... and the stores given that value: S_out = s0 + dt/2 T for the
+// predictors (phase B also keeps T as Tb), T as Tc for phase C with the
+// (Tb T w, T T w) products of the beta partials.
 template <typename CT>
 __device__ __forceinline__ void phase_store(const Args &a, const Member &mb, int comp, size_t o, double T,
                                             double pre, double w, double &p1, double &p2) {
